@@ -1,0 +1,255 @@
+"""fuzzy_ellipses_amd -- MI355X-native fuzzy-metaball rasterizer (host-side Python API).
+
+This mirrors the reference's caller-facing interface for the render path:
+
+* ``render_func_quat`` / ``render_camera``: the arguments of ``baseline/fm_render.py:113``
+  (means, precision factor, log-weights, camera, pose, beta2, beta3) returning depth and
+  alpha images computed by the HIP kernels;
+* ``rasterize``: the argument list of the reference binding
+  ``fuzzy_ellipses/__init__.py:189-236`` (scales/quaternions, camera rays, intrinsics),
+  converting scales/quaternions to the precision factor on the host side and applying the
+  binding's ``nan_to_num`` to the outputs;
+* ``Renderer``: batched forward/backward over many scenes and views in one launch per
+  kernel, with a reusable device workspace.
+
+PyTorch is used only for device memory and the current HIP stream.  All arithmetic runs in
+``lib/libfe_amd.so`` through its C ABI; if that library is missing, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from typing import List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import FeCamera, FeError, FeOptions, FeScene, FeSceneGrad, FeStats, FeView, FeViewCotangent
+from .scenes import BETA2, BETA3, Camera, Scene
+
+__all__ = [
+    "Renderer", "SceneTensors", "render_camera", "render_func_quat", "rasterize", "quat_to_rot",
+    "prec_from_scale_quat", "FeError", "Camera", "Scene", "BETA2", "BETA3",
+]
+
+MODES = {"culled": _lib.FE_MODE_CULLED, "dense": _lib.FE_MODE_DENSE}
+
+
+def _ptr(t: Optional[torch.Tensor]) -> Optional[int]:
+    return None if t is None else t.data_ptr()
+
+
+def _stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+class SceneTensors:
+    """Gaussians of one scene resident on the device (float32, contiguous)."""
+
+    def __init__(self, means, prec, weights_log, device="cuda"):
+        dev = torch.device(device)
+        self.means = torch.as_tensor(means, dtype=torch.float32, device=dev).contiguous()
+        self.prec = torch.as_tensor(prec, dtype=torch.float32, device=dev).contiguous()
+        self.weights_log = torch.as_tensor(weights_log, dtype=torch.float32, device=dev).contiguous()
+        n = self.means.shape[0]
+        if self.means.shape != (n, 3) or self.prec.shape != (n, 3, 3) or self.weights_log.shape != (n,):
+            raise ValueError("SceneTensors: expected means (N,3), prec (N,3,3), weights_log (N,)")
+
+    @classmethod
+    def from_scene(cls, scene: Scene, device="cuda") -> "SceneTensors":
+        return cls(scene.means, scene.prec, scene.weights_log, device)
+
+    @property
+    def n(self) -> int:
+        return int(self.means.shape[0])
+
+    def fe(self) -> FeScene:
+        return FeScene(self.n, _ptr(self.means), _ptr(self.prec), _ptr(self.weights_log))
+
+
+def fe_camera(cam: Camera) -> FeCamera:
+    c = FeCamera()
+    R = np.asarray(cam.R, dtype=np.float32).reshape(9)
+    t = np.asarray(cam.t, dtype=np.float32).reshape(3)
+    for i in range(9):
+        c.R[i] = float(R[i])
+    for i in range(3):
+        c.t[i] = float(t[i])
+    c.fx, c.fy, c.cx, c.cy = float(cam.fx), float(cam.fy), float(cam.cx), float(cam.cy)
+    c.height, c.width = int(cam.H), int(cam.W)
+    return c
+
+
+class Renderer:
+    """Batched renderer.  ``tile`` in {8,16,32}, ``batch`` (LDS depth) in {32,64,128,256}."""
+
+    def __init__(self, tile: int = 16, batch: int = 64, mode: str = "culled", device="cuda"):
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        self.opt = FeOptions(int(tile), int(batch), MODES[mode], 0)
+        self._ws: Optional[torch.Tensor] = None
+        self._last = None  # (scenes, views structs, outputs) of the last forward for backward
+
+    # -- planning -----------------------------------------------------------------------
+    def _structs(self, scenes: Sequence[SceneTensors], views, outs):
+        sc = (FeScene * len(scenes))(*[s.fe() for s in scenes])
+        vw = (FeView * len(views))()
+        for i, (si, cam, b2, b3) in enumerate(views):
+            d, a = outs[i]
+            vw[i] = FeView(int(si), fe_camera(cam), float(b2), float(b3), _ptr(d), _ptr(a))
+        return sc, vw
+
+    def workspace_bytes(self, scenes, views, save_for_backward=False) -> int:
+        outs = [(None, None)] * len(views)
+        sc, vw = self._structs(scenes, views, outs)
+        opt = FeOptions(self.opt.tile, self.opt.batch, self.opt.mode, int(save_for_backward))
+        nbytes = ctypes.c_size_t(0)
+        _lib.check(self.lib.fe_plan(sc, len(scenes), vw, len(views), ctypes.byref(opt), ctypes.byref(nbytes)),
+                   "fe_plan")
+        return int(nbytes.value)
+
+    def _workspace(self, nbytes: int) -> torch.Tensor:
+        if self._ws is None or self._ws.numel() < nbytes:
+            self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
+        return self._ws
+
+    # -- forward ------------------------------------------------------------------------
+    def forward(self, scenes: Sequence[SceneTensors], views, save_for_backward: bool = False,
+                outs=None) -> List[Tuple[torch.Tensor, torch.Tensor]]:
+        """Render every view.  ``views`` = [(scene_index, Camera, beta2, beta3), ...].
+
+        Returns [(depth (H,W), alpha (H,W)), ...] float32 device tensors.
+        """
+        if outs is None:
+            outs = [(torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device),
+                     torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device))
+                    for (_, cam, _, _) in views]
+        sc, vw = self._structs(scenes, views, outs)
+        opt = FeOptions(self.opt.tile, self.opt.batch, self.opt.mode, int(save_for_backward))
+        nbytes = ctypes.c_size_t(0)
+        _lib.check(self.lib.fe_plan(sc, len(scenes), vw, len(views), ctypes.byref(opt), ctypes.byref(nbytes)),
+                   "fe_plan")
+        ws = self._workspace(int(nbytes.value))
+        _lib.check(self.lib.fe_forward(_stream(self.device), sc, len(scenes), vw, len(views), ctypes.byref(opt),
+                                       ws.data_ptr(), ws.numel()), "fe_forward")
+        self._last = (list(scenes), sc, vw, opt, outs) if save_for_backward else None
+        return outs
+
+    def stats(self) -> dict:
+        st = FeStats()
+        _lib.check(self.lib.fe_read_stats(_stream(self.device), self._ws.data_ptr(), self._ws.numel(),
+                                          ctypes.byref(st)), "fe_read_stats")
+        return {"pairs_phase1": st.pairs_phase1, "pairs_phase2": st.pairs_phase2, "evals": st.evals,
+                "floor_tiles": st.floor_tiles}
+
+    # -- backward -----------------------------------------------------------------------
+    def backward(self, d_depth: Sequence[Optional[torch.Tensor]], d_alpha: Sequence[Optional[torch.Tensor]]):
+        """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> with respect to every
+        scene's (means, prec upper triangle, weights_log), from the last forward with
+        save_for_backward=True."""
+        if self._last is None:
+            raise FeError("backward: the last forward was not run with save_for_backward=True")
+        scenes, sc, vw, opt, outs = self._last
+        cot = (FeViewCotangent * len(outs))()
+        keep = []
+        for i in range(len(outs)):
+            dd = None if d_depth[i] is None else d_depth[i].to(torch.float32).contiguous()
+            da = None if d_alpha[i] is None else d_alpha[i].to(torch.float32).contiguous()
+            keep += [dd, da]
+            cot[i] = FeViewCotangent(_ptr(dd), _ptr(da))
+        grads = [(torch.zeros_like(s.means), torch.zeros_like(s.prec), torch.zeros_like(s.weights_log))
+                 for s in scenes]
+        gs = (FeSceneGrad * len(scenes))(*[FeSceneGrad(_ptr(a), _ptr(b), _ptr(c)) for a, b, c in grads])
+        ws = self._ws
+        _lib.check(self.lib.fe_backward(_stream(self.device), sc, len(scenes), vw, len(outs), ctypes.byref(opt),
+                                        cot, gs, ws.data_ptr(), ws.numel()), "fe_backward")
+        return grads
+
+
+# ------------------------------------------------------------------------------------------
+# reference-shaped entry points
+# ------------------------------------------------------------------------------------------
+_default = {}
+
+
+def _renderer(tile=16, batch=64, mode="culled", device="cuda") -> Renderer:
+    key = (tile, batch, mode, str(device))
+    if key not in _default:
+        _default[key] = Renderer(tile, batch, mode, device)
+    return _default[key]
+
+
+def quat_to_rot(q) -> np.ndarray:
+    """Scalar-first quaternion -> rotation (baseline/util_render.py:51-76 semantics)."""
+    w, x, y, z = [float(v) for v in q]
+    nq = w * w + x * x + y * y + z * z
+    if not nq > 1e-12:
+        return np.eye(3)
+    s = 2.0 / nq
+    X, Y, Z = x * s, y * s, z * s
+    return np.array([[1.0 - (y * Y + z * Z), x * Y - w * Z, x * Z + w * Y],
+                     [x * Y + w * Z, 1.0 - (x * X + z * Z), y * Z - w * X],
+                     [x * Z - w * Y, y * Z + w * X, 1.0 - (x * X + y * Y)]])
+
+
+def render_camera(means, prec_full, weights_log, camera: Camera, beta_2=BETA2, beta_3=BETA3,
+                  tile=16, batch=64, mode="culled", device="cuda"):
+    """Depth and alpha (H, W) for one pinhole camera; same math as render_func_rays."""
+    st = means if isinstance(means, SceneTensors) else SceneTensors(means, prec_full, weights_log, device)
+    r = _renderer(tile, batch, mode, device)
+    (d, a), = r.forward([st], [(0, camera, beta_2, beta_3)])
+    return d, a
+
+
+def render_func_quat(means, prec_full, weights_log, intrinsics, quat, t, beta_2=BETA2, beta_3=BETA3, **kw):
+    """baseline/fm_render.py:113-126 with the camera rays given by pinhole intrinsics
+    (H, W, fx, fy, cx, cy) instead of an explicit (H*W, 3) ray array.  Returns
+    (depth (H*W,), alpha (H*W,), None, None) like the reference (the (N, P) weight array
+    is not materialised)."""
+    H, W, fx, fy, cx, cy = intrinsics
+    cam = Camera(R=quat_to_rot(quat), t=np.asarray(t, dtype=np.float64), H=int(H), W=int(W), fx=float(fx),
+                 fy=float(fy), cx=float(cx), cy=float(cy))
+    d, a = render_camera(means, prec_full, weights_log, cam, beta_2, beta_3, **kw)
+    return d.reshape(-1), a.reshape(-1), None, None
+
+
+def prec_from_scale_quat(scales, quats) -> np.ndarray:
+    """Upper-triangular U with U U^T = (R diag(s^2) R^T)^-1 for each Gaussian (float64 host math).
+
+    The reference binding takes scales and quaternions (fuzzy_ellipses/__init__.py:189-203);
+    the renderer's parameter is the precision factor of baseline/fm_render.py:14.
+    """
+    scales = np.asarray(scales, dtype=np.float64)
+    quats = np.asarray(quats, dtype=np.float64)
+    R = np.stack([quat_to_rot(q) for q in quats])
+    cov = np.einsum("nij,nj,nkj->nik", R, scales ** 2, R)
+    C = np.linalg.cholesky(cov)
+    return np.triu(np.transpose(np.linalg.inv(C), (0, 2, 1)))
+
+
+def rasterize(means3D, scales, rotations, weights, camera_rays, camera_rot, camera_trans, image_height,
+              image_width, fx, fy, cx, cy, near=0.01, far=100.0, device="cuda"):
+    """Argument list of the reference binding (fuzzy_ellipses/__init__.py:189-236).
+
+    ``camera_rays`` must be the pinhole grid ((x-cx)/fx, (y-cy)/fy, 1) of the given
+    intrinsics (it is spot-checked); ``near``/``far`` only fed the reference's culling
+    projection and are unused.  Uses the reference's fixed beta2=21.4, beta3=2.66
+    (rasterizer_impl.cu:196-197) and returns nan_to_num'd (depth, alpha) of shape (H, W).
+    """
+    H, W = int(image_height), int(image_width)
+    rays = np.asarray(camera_rays, dtype=np.float64).reshape(-1, 3)
+    if rays.shape[0] != H * W:
+        raise ValueError("rasterize: camera_rays must have H*W rows")
+    idx = np.array([0, W - 1, (H - 1) * W, H * W - 1, (H // 2) * W + W // 2])
+    xs, ys = (idx % W).astype(np.float64), (idx // W).astype(np.float64)
+    expect = np.stack([(xs - cx) / fx, (ys - cy) / fy, np.ones_like(xs)], 1)
+    if not np.allclose(rays[idx], expect, rtol=1e-5, atol=1e-6):
+        raise ValueError("rasterize: camera_rays is not the pinhole grid of (fx, fy, cx, cy)")
+    prec = prec_from_scale_quat(scales, rotations)
+    cam = Camera(R=np.asarray(camera_rot, dtype=np.float64), t=np.asarray(camera_trans, dtype=np.float64),
+                 H=H, W=W, fx=float(fx), fy=float(fy), cx=float(cx), cy=float(cy))
+    d, a = render_camera(np.asarray(means3D), prec, np.asarray(weights), cam, BETA2, BETA3, device=device)
+    d = torch.nan_to_num(d, nan=0.0, posinf=10.0, neginf=0.0)
+    a = torch.nan_to_num(a, nan=0.0, posinf=1.0, neginf=0.0)
+    return d, a

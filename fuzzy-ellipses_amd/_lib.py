@@ -1,0 +1,108 @@
+"""ctypes binding of libfe_amd.so (the C ABI declared in include/fuzzy_ellipses_amd.h).
+
+The library is loaded from this package's lib/ directory.  There is no fallback: if the
+library is missing or does not export the full ABI, every render call raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libfe_amd.so")
+
+FE_OK = 0
+FE_MODE_CULLED = 0
+FE_MODE_DENSE = 1
+
+EXPORTS = [
+    "fe_version", "fe_status_string", "fe_last_error_string", "fe_plan", "fe_forward",
+    "fe_backward", "fe_read_stats", "fe_render_workspace_bytes", "fe_render_fwd",
+]
+
+
+class FeCamera(ctypes.Structure):
+    _fields_ = [("R", ctypes.c_float * 9), ("t", ctypes.c_float * 3), ("fx", ctypes.c_float),
+                ("fy", ctypes.c_float), ("cx", ctypes.c_float), ("cy", ctypes.c_float),
+                ("height", ctypes.c_int32), ("width", ctypes.c_int32)]
+
+
+class FeScene(ctypes.Structure):
+    _fields_ = [("n", ctypes.c_int32), ("means", ctypes.c_void_p), ("prec", ctypes.c_void_p),
+                ("weights_log", ctypes.c_void_p)]
+
+
+class FeView(ctypes.Structure):
+    _fields_ = [("scene", ctypes.c_int32), ("cam", FeCamera), ("beta2", ctypes.c_float),
+                ("beta3", ctypes.c_float), ("depth", ctypes.c_void_p), ("alpha", ctypes.c_void_p)]
+
+
+class FeSceneGrad(ctypes.Structure):
+    _fields_ = [("d_means", ctypes.c_void_p), ("d_prec", ctypes.c_void_p),
+                ("d_weights_log", ctypes.c_void_p)]
+
+
+class FeViewCotangent(ctypes.Structure):
+    _fields_ = [("d_depth", ctypes.c_void_p), ("d_alpha", ctypes.c_void_p)]
+
+
+class FeOptions(ctypes.Structure):
+    _fields_ = [("tile", ctypes.c_int32), ("batch", ctypes.c_int32), ("mode", ctypes.c_int32),
+                ("save_for_backward", ctypes.c_int32)]
+
+
+class FeStats(ctypes.Structure):
+    _fields_ = [("pairs_phase1", ctypes.c_int64), ("pairs_phase2", ctypes.c_int64),
+                ("evals", ctypes.c_int64), ("floor_tiles", ctypes.c_int64)]
+
+
+class FeError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the HIP library once; raise FeError if it is missing or incomplete."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FeError(f"HIP library not built: {path} is missing (run __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    missing = [s for s in EXPORTS if not hasattr(lib, s)]
+    if missing:
+        raise FeError(f"{path} does not export {missing}")
+    lib.fe_version.restype = ctypes.c_int32
+    lib.fe_status_string.restype = ctypes.c_char_p
+    lib.fe_status_string.argtypes = [ctypes.c_int]
+    lib.fe_last_error_string.restype = ctypes.c_char_p
+    lib.fe_plan.restype = ctypes.c_int
+    lib.fe_plan.argtypes = [ctypes.POINTER(FeScene), ctypes.c_int32, ctypes.POINTER(FeView), ctypes.c_int32,
+                            ctypes.POINTER(FeOptions), ctypes.POINTER(ctypes.c_size_t)]
+    lib.fe_forward.restype = ctypes.c_int
+    lib.fe_forward.argtypes = [ctypes.c_void_p, ctypes.POINTER(FeScene), ctypes.c_int32, ctypes.POINTER(FeView),
+                               ctypes.c_int32, ctypes.POINTER(FeOptions), ctypes.c_void_p, ctypes.c_size_t]
+    lib.fe_backward.restype = ctypes.c_int
+    lib.fe_backward.argtypes = [ctypes.c_void_p, ctypes.POINTER(FeScene), ctypes.c_int32, ctypes.POINTER(FeView),
+                                ctypes.c_int32, ctypes.POINTER(FeOptions), ctypes.POINTER(FeViewCotangent),
+                                ctypes.POINTER(FeSceneGrad), ctypes.c_void_p, ctypes.c_size_t]
+    lib.fe_read_stats.restype = ctypes.c_int
+    lib.fe_read_stats.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(FeStats)]
+    lib.fe_render_workspace_bytes.restype = ctypes.c_int
+    lib.fe_render_workspace_bytes.argtypes = [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_size_t)]
+    lib.fe_render_fwd.restype = ctypes.c_int
+    lib.fe_render_fwd.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                  ctypes.c_void_p, ctypes.POINTER(FeCamera), ctypes.c_float, ctypes.c_float,
+                                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    _lib = lib
+    return lib
+
+
+def check(status: int, what: str = "") -> None:
+    if status != FE_OK:
+        lib = load()
+        raise FeError(f"{what}: {lib.fe_status_string(status).decode()} "
+                      f"({lib.fe_last_error_string().decode()})")

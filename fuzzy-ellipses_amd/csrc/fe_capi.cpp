@@ -1,0 +1,364 @@
+// C ABI of the MI355X fuzzy-metaball rasterizer: planning, forward and backward entry points.
+//
+// The host side only does O(views) arithmetic: it lays out one caller-provided workspace
+// (sized for the worst case so nothing is ever read back), uploads the job descriptors and
+// enqueues the kernels on the caller's stream.  No host synchronisation on the render path.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "../../include/fuzzy_ellipses_amd.h"
+#include "fe_common.h"
+#include "fe_launch.h"
+
+namespace fe {
+
+static thread_local char g_err[512] = "";
+
+static void set_err(const char* what, hipError_t e) {
+  snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
+}
+static void set_msg(const char* what) { snprintf(g_err, sizeof(g_err), "%s", what); }
+
+#define FE_HIP(call)                         \
+  do {                                       \
+    hipError_t e_ = (call);                  \
+    if (e_ != hipSuccess) {                  \
+      ::fe::set_err(#call, e_);              \
+      return FE_ERR_HIP;                     \
+    }                                        \
+  } while (0)
+
+static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+// Workspace layout.  Every region is 256-byte aligned.
+struct Layout {
+  int n_jobs = 0;
+  int max_n = 0;
+  int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0;
+  int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
+  size_t o_header = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_mask = 0, o_count = 0,
+         o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
+         o_gacc = 0, bytes = 0;
+};
+
+// A small header at the start of the workspace lets fe_read_stats find the counters.
+struct WsHeader {
+  uint64_t magic;
+  uint64_t o_counters, o_off1, o_off2, total_tiles, mode;
+};
+constexpr uint64_t kMagic = 0x4645414d44313031ull;  // "FEAMD101"
+
+static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* views, int32_t n_views,
+                      const fe_options* opt, Layout* L, std::vector<Job>* jobs) {
+  if (!scenes || !views || n_scenes <= 0 || n_views <= 0) {
+    set_msg("fe_plan: need at least one scene and one view");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  fe_options o = {16, 64, FE_MODE_CULLED, 0};
+  if (opt) o = *opt;
+  if (o.tile == 0) o.tile = 16;
+  if (o.batch == 0) o.batch = 64;
+  if (o.tile != 8 && o.tile != 16 && o.tile != 32) {
+    set_msg("fe_plan: tile must be 8, 16 or 32");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  if (o.batch != 32 && o.batch != 64 && o.batch != 128 && o.batch != 256) {
+    set_msg("fe_plan: batch must be 32, 64, 128 or 256");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  if (o.mode != FE_MODE_CULLED && o.mode != FE_MODE_DENSE) {
+    set_msg("fe_plan: unknown mode");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  L->ts = o.tile;
+  L->gb = o.batch;
+  L->mode = o.mode;
+  L->save = o.save_for_backward ? 1 : 0;
+  L->n_jobs = n_views;
+  if (jobs) jobs->resize(n_views);
+  int64_t rec = 0, tiles = 0, maskw = 0, pix = 0, cap = 0;
+  int max_n = 0;
+  for (int v = 0; v < n_views; ++v) {
+    const fe_view& vw = views[v];
+    if (vw.scene < 0 || vw.scene >= n_scenes) {
+      set_msg("fe_plan: view references a scene out of range");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+    const fe_scene& sc = scenes[vw.scene];
+    const fe_camera& c = vw.cam;
+    if (sc.n < 0 || c.height <= 0 || c.width <= 0 || !(c.fx != 0.0f) || !(c.fy != 0.0f)) {
+      set_msg("fe_plan: invalid scene size or camera");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+    if ((int64_t)c.height * c.width > (int64_t)1 << 30 || sc.n > (1 << 26)) {
+      set_msg("fe_plan: image or scene too large");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+    const int tx = (c.width + o.tile - 1) / o.tile, ty = (c.height + o.tile - 1) / o.tile;
+    const int nw = (sc.n + 63) / 64;
+    if (jobs) {
+      Job& jb = (*jobs)[v];
+      memset(&jb, 0, sizeof(Job));
+      jb.means = sc.means;
+      jb.prec = sc.prec;
+      jb.wlog = sc.weights_log;
+      jb.depth = vw.depth;
+      jb.alpha = vw.alpha;
+      memcpy(jb.R, c.R, sizeof(jb.R));
+      memcpy(jb.o, c.t, sizeof(jb.o));
+      jb.fx = c.fx; jb.fy = c.fy; jb.cx = c.cx; jb.cy = c.cy;
+      jb.beta2 = vw.beta2; jb.beta3 = vw.beta3;
+      const double ln_n = log((double)std::max(sc.n, 1));
+      // alpha: total dropped density <= n * exp(-c_alpha) = exp(-14) ~ 8e-7
+      jb.c_alpha = (float)(ln_n + 14.0);
+      // depth: dropped softmax weight fraction <= n * exp(-delta) = exp(-20) ~ 2e-9
+      jb.delta2 = (float)((ln_n + 20.0) * 1.4426950408889634);
+      jb.n = sc.n; jb.H = c.height; jb.W = c.width;
+      jb.tiles_x = tx; jb.tiles_y = ty; jb.n_tiles = tx * ty;
+      jb.nw = nw;
+      jb.scene = vw.scene;
+      jb.rec_base = rec;
+      jb.tile_base = tiles;
+      jb.mask_base = maskw;
+      jb.pix_base = pix;
+    }
+    max_n = std::max(max_n, sc.n);
+    rec += sc.n;
+    tiles += (int64_t)tx * ty;
+    maskw += (int64_t)tx * ty * nw;
+    pix += (int64_t)c.height * c.width;
+    cap += (int64_t)tx * ty * sc.n;
+  }
+  if (cap > ((int64_t)1 << 34)) {
+    set_msg("fe_plan: batch too large (tile-pair capacity above 2^34)");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  L->max_n = max_n;
+  L->total_rec = rec;
+  L->total_tiles = tiles;
+  L->total_mask = maskw;
+  L->total_pix = pix;
+  L->list_cap = cap;
+  size_t p = 0;
+  L->o_header = p; p = align256(p + sizeof(WsHeader));
+  L->o_jobs = p; p = align256(p + sizeof(Job) * n_views);
+  L->o_counters = p; p = align256(p + sizeof(Counters));
+  L->o_flags = p; p = align256(p + sizeof(int32_t) * n_views);
+  L->o_rec = p; p = align256(p + sizeof(float) * kRecFloats * (size_t)std::max<int64_t>(rec, 1));
+  L->o_lmin = p; p = align256(p + sizeof(float) * tiles);
+  L->o_ubm = p; p = align256(p + sizeof(float) * tiles);
+  L->o_state = p; p = align256(p + sizeof(float) * 6 * pix);
+  if (o.mode == FE_MODE_CULLED) {
+    L->o_mask = p; p = align256(p + sizeof(unsigned long long) * std::max<int64_t>(maskw, 1));
+    L->o_count = p; p = align256(p + sizeof(uint32_t) * tiles);
+    L->o_off1 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
+    L->o_off2 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
+    L->o_partial = p; p = align256(p + sizeof(unsigned long long) * (scan_partial_count(tiles) + 1));
+    L->o_list = p; p = align256(p + sizeof(uint32_t) * (size_t)std::max<int64_t>(cap, 1));
+  }
+  if (L->save) {
+    L->o_gacc = p; p = align256(p + sizeof(float) * 16 * (size_t)std::max<int64_t>(rec, 1));
+  }
+  L->bytes = p;
+  return FE_OK;
+}
+
+// Pinned staging for the job descriptors, one per device.  Before overwriting, wait for the
+// previous upload from it (normally long finished).
+struct Staging {
+  std::mutex mu;
+  void* host = nullptr;
+  size_t cap = 0;
+  hipEvent_t ev = nullptr;
+  bool pending = false;
+};
+static Staging g_staging[64];
+
+static fe_status upload(hipStream_t s, void* dst, const void* src, size_t bytes) {
+  int dev = 0;
+  FE_HIP(hipGetDevice(&dev));
+  Staging& st = g_staging[dev & 63];
+  std::lock_guard<std::mutex> lk(st.mu);
+  if (st.pending) {
+    FE_HIP(hipEventSynchronize(st.ev));
+    st.pending = false;
+  }
+  if (!st.ev) FE_HIP(hipEventCreateWithFlags(&st.ev, hipEventDisableTiming));
+  if (st.cap < bytes) {
+    if (st.host) FE_HIP(hipHostFree(st.host));
+    st.host = nullptr;
+    size_t c = std::max<size_t>(bytes, 1 << 16);
+    FE_HIP(hipHostMalloc(&st.host, c, hipHostMallocDefault));
+    st.cap = c;
+  }
+  memcpy(st.host, src, bytes);
+  FE_HIP(hipMemcpyAsync(dst, st.host, bytes, hipMemcpyHostToDevice, s));
+  FE_HIP(hipEventRecord(st.ev, s));
+  st.pending = true;
+  return FE_OK;
+}
+
+template <typename T>
+static T* at(void* ws, size_t off) {
+  return reinterpret_cast<T*>(reinterpret_cast<char*>(ws) + off);
+}
+
+fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
+                  int32_t n_views, const fe_options* opt, void* ws, size_t ws_bytes) {
+  Layout L;
+  std::vector<Job> jobs;
+  fe_status st = plan(scenes, n_scenes, views, n_views, opt, &L, &jobs);
+  if (st != FE_OK) return st;
+  if (!ws || ws_bytes < L.bytes) {
+    set_msg("fe_forward: workspace missing or too small (see fe_plan)");
+    return FE_ERR_WORKSPACE;
+  }
+  for (int v = 0; v < n_views; ++v) {
+    if (!views[v].depth || !views[v].alpha) {
+      set_msg("fe_forward: null output buffer");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+    const fe_scene& sc = scenes[views[v].scene];
+    if (sc.n > 0 && (!sc.means || !sc.prec || !sc.weights_log)) {
+      set_msg("fe_forward: null scene buffer");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+  }
+  // header + jobs in one upload
+  std::vector<char> blob(L.o_counters);
+  WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode};
+  memcpy(blob.data() + L.o_header, &h, sizeof(h));
+  memcpy(blob.data() + L.o_jobs, jobs.data(), sizeof(Job) * n_views);
+  st = upload(s, ws, blob.data(), blob.size());
+  if (st != FE_OK) return st;
+
+  const Job* dj = at<Job>(ws, L.o_jobs);
+  Counters* cnt = at<Counters>(ws, L.o_counters);
+  int32_t* flags = at<int32_t>(ws, L.o_flags);
+  float* rec = at<float>(ws, L.o_rec);
+  float* state = at<float>(ws, L.o_state);
+  float* lmin = at<float>(ws, L.o_lmin);
+  float* ubm = at<float>(ws, L.o_ubm);
+  FE_HIP(launch_init(s, cnt, flags, L.n_jobs));
+  FE_HIP(launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, flags));
+  if (L.mode == FE_MODE_DENSE) {
+    FE_HIP(launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, nullptr, nullptr, nullptr,
+                        state, L.total_pix, lmin, ubm, cnt, L.save));
+    return FE_OK;
+  }
+  unsigned long long* mask = at<unsigned long long>(ws, L.o_mask);
+  uint32_t* count = at<uint32_t>(ws, L.o_count);
+  unsigned long long* off1 = at<unsigned long long>(ws, L.o_off1);
+  unsigned long long* off2 = at<unsigned long long>(ws, L.o_off2);
+  unsigned long long* partial = at<unsigned long long>(ws, L.o_partial);
+  uint32_t* list = at<uint32_t>(ws, L.o_list);
+  // phase 1
+  FE_HIP(launch_tile_mask(s, 1, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, count, lmin, ubm, flags));
+  FE_HIP(launch_scan(s, count, L.total_tiles, partial, off1, nullptr));
+  FE_HIP(launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list));
+  FE_HIP(launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state,
+                      L.total_pix, lmin, ubm, cnt, L.save));
+  // phase 2
+  FE_HIP(launch_tile_mask(s, 2, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, count, lmin, ubm, flags));
+  FE_HIP(launch_scan(s, count, L.total_tiles, partial, off2, off1 + L.total_tiles));
+  FE_HIP(launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off2, list));
+  FE_HIP(launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state,
+                      L.total_pix, lmin, ubm, cnt, L.save));
+  return FE_OK;
+}
+
+}  // namespace fe
+
+extern "C" {
+
+int32_t fe_version(void) { return 101; }
+
+const char* fe_status_string(fe_status s) {
+  switch (s) {
+    case FE_OK: return "ok";
+    case FE_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case FE_ERR_WORKSPACE: return "workspace missing or too small";
+    case FE_ERR_HIP: return "HIP runtime error";
+    case FE_ERR_NO_DEVICE: return "no device";
+  }
+  return "unknown status";
+}
+
+const char* fe_last_error_string(void) { return fe::g_err; }
+
+fe_status fe_plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* views, int32_t n_views,
+                  const fe_options* opt, size_t* workspace_bytes) {
+  if (!workspace_bytes) return FE_ERR_INVALID_ARGUMENT;
+  fe::Layout L;
+  fe_status st = fe::plan(scenes, n_scenes, views, n_views, opt, &L, nullptr);
+  if (st == FE_OK) *workspace_bytes = L.bytes;
+  return st;
+}
+
+fe_status fe_forward(void* stream, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
+                     int32_t n_views, const fe_options* opt, void* workspace, size_t workspace_bytes) {
+  return fe::forward(reinterpret_cast<hipStream_t>(stream), scenes, n_scenes, views, n_views, opt,
+                     workspace, workspace_bytes);
+}
+
+fe_status fe_read_stats(void* stream, const void* workspace, size_t workspace_bytes, fe_stats* out) {
+  if (!workspace || !out || workspace_bytes < sizeof(fe::WsHeader)) return FE_ERR_INVALID_ARGUMENT;
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  fe::WsHeader h;
+  FE_HIP(hipMemcpyAsync(&h, workspace, sizeof(h), hipMemcpyDeviceToHost, s));
+  FE_HIP(hipStreamSynchronize(s));
+  if (h.magic != fe::kMagic) {
+    fe::set_msg("fe_read_stats: workspace was not written by fe_forward");
+    return FE_ERR_WORKSPACE;
+  }
+  fe::Counters c;
+  const char* base = reinterpret_cast<const char*>(workspace);
+  FE_HIP(hipMemcpyAsync(&c, base + h.o_counters, sizeof(c), hipMemcpyDeviceToHost, s));
+  unsigned long long t1 = 0, t2 = 0;
+  if (h.mode == FE_MODE_CULLED) {
+    FE_HIP(hipMemcpyAsync(&t1, base + h.o_off1 + 8 * h.total_tiles, 8, hipMemcpyDeviceToHost, s));
+    FE_HIP(hipMemcpyAsync(&t2, base + h.o_off2 + 8 * h.total_tiles, 8, hipMemcpyDeviceToHost, s));
+  }
+  FE_HIP(hipStreamSynchronize(s));
+  out->pairs_phase1 = (int64_t)t1;
+  out->pairs_phase2 = (int64_t)(t2 - t1);
+  out->evals = (int64_t)c.evals;
+  out->floor_tiles = (int64_t)c.floor_tiles;
+  return FE_OK;
+}
+
+fe_status fe_render_workspace_bytes(int32_t n, int32_t height, int32_t width, int32_t tile, size_t* bytes) {
+  fe_scene sc{n, nullptr, nullptr, nullptr};
+  fe_view v;
+  memset(&v, 0, sizeof(v));
+  v.cam.height = height;
+  v.cam.width = width;
+  v.cam.fx = v.cam.fy = 1.0f;
+  fe_options o{tile, 64, FE_MODE_CULLED, 0};
+  return fe_plan(&sc, 1, &v, 1, &o, bytes);
+}
+
+fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float* prec,
+                        const float* weights_log, const fe_camera* cam, float beta2, float beta3,
+                        float* depth, float* alpha, void* workspace, size_t workspace_bytes) {
+  if (!cam) return FE_ERR_INVALID_ARGUMENT;
+  fe_scene sc{n, means, prec, weights_log};
+  fe_view v;
+  v.scene = 0;
+  v.cam = *cam;
+  v.beta2 = beta2;
+  v.beta3 = beta3;
+  v.depth = depth;
+  v.alpha = alpha;
+  fe_options o{16, 64, FE_MODE_CULLED, 0};
+  return fe_forward(stream, &sc, 1, &v, 1, &o, workspace, workspace_bytes);
+}
+
+}  // extern "C"

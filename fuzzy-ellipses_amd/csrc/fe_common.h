@@ -1,0 +1,157 @@
+// Shared device-side definitions of the fuzzy-metaball rasterizer (gfx950).
+//
+// Per (Gaussian, view) record, 16 floats (64 B, one cache-line quarter, four float4):
+//   [0..2]  a = M[:,0] / fx      M = A R^T, A = triu(prec)^T (lower), R rows = camera axes
+//   [3..5]  b = M[:,1] / fy
+//   [6..8]  g = M[:,2]
+//   [9..11] q = A (mu - o)
+//   [12]    w        log-weight
+//   [13]    w*log2(e)
+//   [14]    q.q
+//   [15]    flags (bit 0: degenerate record -- non-finite input or a zero diagonal of U)
+// With these, the whitened ray of pixel (x, y) is u = a*(x-cx) + b*(y-cy) + g, which equals
+// triu(prec)^T (r_cam @ R) of baseline/fm_render.py:33-52, and per pair
+//   t* = (q.u)/(u.u),  d0 = |t* u - q|^2,  d2 = w - d0/2   (fm_render.py:53-70).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace fe {
+
+constexpr float kLog2e = 1.4426950408889634f;
+constexpr float kLn2 = 0.6931471805599453f;
+constexpr int kRecFloats = 16;
+
+// One view of one scene, as the kernels see it.  Built on the host by the planner.
+struct Job {
+  const float* means;
+  const float* prec;
+  const float* wlog;
+  float* depth;
+  float* alpha;
+  float R[9];
+  float o[3];
+  float fx, fy, cx, cy;
+  float beta2, beta3;
+  float c_alpha;   // alpha cull constant, natural-log units: drop if d2 < -c_alpha on a tile
+  float delta2;    // depth cull margin, log2 units
+  int32_t n, H, W;
+  int32_t tiles_x, tiles_y, n_tiles;
+  int32_t nw;      // 64-bit mask words per tile = ceil(n / 64)
+  int32_t scene;
+  int64_t rec_base;   // first record (Gaussian-view) of this job
+  int64_t tile_base;  // first global tile
+  int64_t mask_base;  // first mask word
+  int64_t pix_base;   // first pixel of this view in the per-pixel state arrays
+};
+
+// Device counters (zeroed by k_init each forward).
+struct Counters {
+  unsigned long long evals;
+  unsigned long long floor_tiles;
+  unsigned long long pad[6];
+};
+
+__device__ __forceinline__ float dot3(const float* x, const float* y) {
+  return fmaf(x[0], y[0], fmaf(x[1], y[1], x[2] * y[2]));
+}
+
+// c = x cross y
+__device__ __forceinline__ void cross3(const float* x, const float* y, float* c) {
+  c[0] = fmaf(x[1], y[2], -(x[2] * y[1]));
+  c[1] = fmaf(x[2], y[0], -(x[0] * y[2]));
+  c[2] = fmaf(x[0], y[1], -(x[1] * y[0]));
+}
+
+// Binary search: the job whose [base, next base) range holds idx.  `bases` are read through
+// the Job array (tile_base or rec_base selected by the accessor).
+template <typename F>
+__device__ __forceinline__ int find_job(int n_jobs, int64_t idx, F base_of) {
+  int lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (base_of(mid) <= idx) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// Tile geometry in centred pixel coordinates: centre (Xc, Yc), half extents (hx, hy) over
+// the tile's pixel centres, clipped to the image.
+struct TileGeo {
+  float Xc, Yc, hx, hy;
+};
+
+__device__ __forceinline__ TileGeo tile_geo(const Job& jb, int tile, int ts) {
+  int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
+  int x0 = tx * ts, y0 = ty * ts;
+  int xl = min(x0 + ts, jb.W) - 1, yl = min(y0 + ts, jb.H) - 1;
+  TileGeo g;
+  g.Xc = (float)(x0 + xl) * 0.5f - jb.cx;
+  g.Yc = (float)(y0 + yl) * 0.5f - jb.cy;
+  g.hx = (float)(xl - x0) * 0.5f;
+  g.hy = (float)(yl - y0) * 0.5f;
+  return g;
+}
+
+// Conservative bounds of one Gaussian over one tile's pixel rays.
+//   sd_lo, sd_hi : bounds of sqrt(d0) (distance of q to the ray line in whitened space)
+//   t_lb         : lower bound of t* where the whole tile lies on the q side (else 0)
+//   plus         : true when every ray of the tile has q.u > 0 (t* > 0 on the tile)
+//   t_any_lb     : lower bound of t* on the tile regardless of side (-inf if unknown)
+//   unbounded    : the tile cone is too wide for the bound (keep the pair)
+// sqrt(d0) = |q| sin(angle(q, line)) is |q|-Lipschitz in the ray angle; the angle from the
+// centre ray to any tile ray is at most the largest corner angle, bounded by its tangent.
+struct PairBound {
+  float sd_lo, sd_hi, t_lb, t_any_lb;
+  bool plus, unbounded;
+};
+
+__device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& tg) {
+  const float* a = r;
+  const float* b = r + 3;
+  const float* g = r + 6;
+  const float* q = r + 9;
+  const float qq = r[14];
+  float uc[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) uc[i] = fmaf(a[i], tg.Xc, fmaf(b[i], tg.Yc, g[i]));
+  const float uu = dot3(uc, uc);
+  const float qu = dot3(q, uc);
+  const float tc = qu / uu;
+  float v[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) v[i] = fmaf(tc, uc[i], -q[i]);
+  const float sdc = sqrtf(dot3(v, v));
+  const float ua = dot3(uc, a);
+  const float ub = dot3(uc, b);
+  float ca[3], cb[3], pp[3], pm[3], ap[3], am[3];
+  cross3(uc, a, ca);
+  cross3(uc, b, cb);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    pp[i] = fmaf(ca[i], tg.hx, cb[i] * tg.hy);
+    pm[i] = fmaf(ca[i], tg.hx, -(cb[i] * tg.hy));
+    ap[i] = fmaf(a[i], tg.hx, b[i] * tg.hy);
+    am[i] = fmaf(a[i], tg.hx, -(b[i] * tg.hy));
+  }
+  const float n1 = sqrtf(dot3(pp, pp));
+  const float n2 = sqrtf(dot3(pm, pm));
+  const float den = fmaf(-fabsf(ub), tg.hy, fmaf(-fabsf(ua), tg.hx, uu));
+  PairBound pb;
+  pb.unbounded = !(den > 0.0f);
+  const float tanm = fmaxf(n1, n2) / den;
+  const float qn = sqrtf(qq);
+  const float dth = qn * tanm;
+  pb.sd_lo = fmaxf(sdc - dth, 0.0f);
+  pb.sd_hi = sdc + dth;
+  const float rho = sqrtf(fmaxf(dot3(ap, ap), dot3(am, am)));
+  const float un = sqrtf(uu);
+  pb.plus = (qu > 0.0f) && (pb.sd_hi < qn);
+  pb.t_lb = pb.plus ? sqrtf(fmaxf(fmaf(-pb.sd_hi, pb.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
+  pb.t_any_lb = (un > rho) ? -(qn / (un - rho)) : -INFINITY;
+  if (!(pb.sd_lo == pb.sd_lo)) pb.unbounded = true;  // NaN anywhere -> keep
+  return pb;
+}
+
+}  // namespace fe

@@ -1,0 +1,557 @@
+// Forward kernels of the fuzzy-metaball rasterizer for MI355X (gfx950, wave64).
+//
+// Pipeline per batch of views (one launch of each kernel covers every view of the batch):
+//   k_init        zero counters / per-job flags (a kernel, not hipMemset: graph-safe)
+//   k_preprocess  (a) per (Gaussian, view): whitened camera-frame record (fe_common.h)
+//   k_tile_mask   (b) per (tile, Gaussian): conservative bound test, wave ballot -> 64-bit
+//                     membership words + per-tile histogram (count)
+//   k_scan_*          exclusive prefix sum of the tile histogram -> tile ranges
+//   k_tile_write      in-order compaction of the membership words -> per-tile Gaussian lists
+//                     (tile-major emission makes the list sorted by (tile, gaussian id):
+//                     this is the counting sort on the tile digit)
+//   k_blend       (c) per tile: LDS-staged batches of records, per pixel ray the max-density
+//                     point t*, density, alpha sum and the online-max depth softmax
+// Culled mode runs (b)+(c) twice: phase 1 lists pairs whose density can reach the alpha
+// threshold; the phase-1 blend yields, per tile, the exact minimum over its pixels of the
+// log-sum-exp of the listed depth logits; phase 2 adds every pair whose logit upper bound is
+// within the depth margin of it and finishes the pixels.  Dense mode evaluates all pairs.
+#include "fe_common.h"
+
+namespace fe {
+
+// ------------------------------------------------------------------------------------------
+// small helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ float nan_to_num(float t) {
+  // jnp.nan_to_num: NaN -> 0, +-inf -> +-FLT_MAX
+  return (t == t) ? fminf(fmaxf(t, -3.402823466e38f), 3.402823466e38f) : 0.0f;
+}
+
+template <int NT>
+__device__ __forceinline__ float block_min(float v, float* sh) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (NT > 64) {
+    if (lane == 0) sh[wave] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < NT / 64; ++w) v = fminf(v, sh[w]);
+      sh[0] = v;
+    }
+    __syncthreads();
+    v = sh[0];
+  }
+  return v;
+}
+
+// ------------------------------------------------------------------------------------------
+// k_init
+// ------------------------------------------------------------------------------------------
+__global__ void k_init(Counters* cnt, int32_t* job_flags, int n_jobs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < (int)(sizeof(Counters) / 8)) reinterpret_cast<unsigned long long*>(cnt)[i] = 0ull;
+  for (int k = i; k < n_jobs; k += gridDim.x * blockDim.x) job_flags[k] = 0;
+}
+
+// ------------------------------------------------------------------------------------------
+// (a) preprocess: grid (ceil(max_n/256), n_jobs), one thread per Gaussian of the job.
+//     Reads means (N,3), prec (N,3,3) (upper triangle), w (N,) -- a wave reads one
+//     contiguous span of each array.
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs,
+                                                    float* __restrict__ rec,
+                                                    int32_t* __restrict__ job_flags) {
+  const Job& jb = jobs[blockIdx.y];
+  const int j = blockIdx.x * 256 + threadIdx.x;
+  if (j >= jb.n) return;
+  const float* mu = jb.means + 3 * (int64_t)j;
+  const float* P = jb.prec + 9 * (int64_t)j;
+  const float u00 = P[0], u01 = P[1], u02 = P[2], u11 = P[4], u12 = P[5], u22 = P[8];
+  const float w = jb.wlog[j];
+  const float p0 = mu[0] - jb.o[0], p1 = mu[1] - jb.o[1], p2 = mu[2] - jb.o[2];
+  // M = A R^T with A = U^T (lower): M[i][k] = A[i] . R[k]
+  float M[3][3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    const float r0 = jb.R[3 * k + 0], r1 = jb.R[3 * k + 1], r2 = jb.R[3 * k + 2];
+    M[0][k] = u00 * r0;
+    M[1][k] = fmaf(u01, r0, u11 * r1);
+    M[2][k] = fmaf(u02, r0, fmaf(u12, r1, u22 * r2));
+  }
+  float out[16];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    out[0 + i] = M[i][0] / jb.fx;
+    out[3 + i] = M[i][1] / jb.fy;
+    out[6 + i] = M[i][2];
+  }
+  out[9] = u00 * p0;
+  out[10] = fmaf(u01, p0, u11 * p1);
+  out[11] = fmaf(u02, p0, fmaf(u12, p1, u22 * p2));
+  out[12] = w;
+  out[13] = w * kLog2e;
+  out[14] = dot3(out + 9, out + 9);
+  bool finite = true;
+#pragma unroll
+  for (int i = 0; i < 15; ++i) finite = finite && isfinite(out[i]);
+  const bool degenerate = !finite || u00 == 0.0f || u11 == 0.0f || u22 == 0.0f;
+  out[15] = __int_as_float(degenerate ? 1 : 0);
+  if (degenerate) atomicOr(job_flags + blockIdx.y, 1);
+  float4* dst = reinterpret_cast<float4*>(rec + (jb.rec_base + j) * kRecFloats);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) dst[i] = make_float4(out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+}
+
+// ------------------------------------------------------------------------------------------
+// (b) tile membership: one block (4 waves) per tile, each wave tests 64 Gaussians at a time.
+// PHASE 1 keeps pairs whose density bound reaches exp(-c_alpha):  d0_lo/2 - w <= c_alpha.
+// PHASE 2 keeps pairs not kept before whose depth-logit bound reaches the tile threshold
+// tau = min_pixels(log2-sum-exp of phase-1 logits) - delta2, and records the largest logit
+// bound of dropped pairs that might lie behind the camera (for the weight-floor check).
+// ------------------------------------------------------------------------------------------
+template <int PHASE>
+__global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs, int n_jobs, int ts,
+                                                   const float* __restrict__ rec,
+                                                   unsigned long long* __restrict__ mask,
+                                                   uint32_t* __restrict__ count,
+                                                   const float* __restrict__ tile_lmin,
+                                                   float* __restrict__ tile_ubm,
+                                                   const int32_t* __restrict__ job_flags) {
+  __shared__ uint32_t sh_cnt[4];
+  __shared__ float sh_ubm[4];
+  const int64_t gt = blockIdx.x;
+  const int ji = find_job(n_jobs, gt, [&](int k) { return jobs[k].tile_base; });
+  const Job& jb = jobs[ji];
+  const int tile = (int)(gt - jb.tile_base);
+  const TileGeo tg = tile_geo(jb, tile, ts);
+  const bool dense_job = (job_flags[ji] & 1) != 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
+  const float tau = (PHASE == 2) ? tile_lmin[gt] - jb.delta2 : 0.0f;
+  uint32_t cnt = 0;
+  float ubm = -INFINITY;
+  unsigned long long* mrow = mask + jb.mask_base + (int64_t)tile * jb.nw;
+  for (int c = wave; c < jb.nw; c += 4) {
+    const int j = c * 64 + lane;
+    unsigned long long old = 0ull;
+    if (PHASE == 2) old = mrow[c];
+    bool keep = false;
+    const bool listed = (PHASE == 2) && ((old >> lane) & 1ull);
+    if (j < jb.n && !listed) {
+      const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
+      float r[16];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const float4 t4 = r4[i];
+        r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
+      }
+      const bool degenerate = (__float_as_int(r[15]) & 1) != 0;
+      if (dense_job || degenerate) {
+        keep = (PHASE == 1);
+      } else {
+        const PairBound pb = pair_bound(r, tg);
+        const float w = r[12];
+        if (PHASE == 1) {
+          keep = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
+        } else {
+          const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
+          const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
+          keep = pb.unbounded || (ub >= tau);
+          if (!keep && !pb.plus) ubm = fmaxf(ubm, fmaf(b3l, dlo, -(b2l * pb.t_any_lb)));
+        }
+      }
+    }
+    const unsigned long long word = __ballot(keep);
+    if (lane == 0) mrow[c] = word;
+    cnt += (uint32_t)__popcll(word);
+  }
+  if (PHASE == 2) {
+    for (int o = 32; o > 0; o >>= 1) ubm = fmaxf(ubm, __shfl_xor(ubm, o, 64));
+  }
+  if (lane == 0) { sh_cnt[wave] = cnt; sh_ubm[wave] = ubm; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    count[gt] = sh_cnt[0] + sh_cnt[1] + sh_cnt[2] + sh_cnt[3];
+    if (PHASE == 2) tile_ubm[gt] = fmaxf(fmaxf(sh_ubm[0], sh_ubm[1]), fmaxf(sh_ubm[2], sh_ubm[3]));
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// exclusive prefix sum: uint32 counts -> uint64 offsets (off[n] = base + total)
+// ------------------------------------------------------------------------------------------
+constexpr int kScanBlock = 1024;  // items per block (256 threads x 4)
+
+__device__ __forceinline__ unsigned long long wave_incl_scan(unsigned long long v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned long long y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  return v;
+}
+
+// block-wide exclusive scan over 256 threads; returns exclusive prefix, total via *tot
+__device__ __forceinline__ unsigned long long block_excl_scan256(unsigned long long v,
+                                                                 unsigned long long* sh,
+                                                                 unsigned long long* tot) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  unsigned long long inc = wave_incl_scan(v);
+  if (lane == 63) sh[wave] = inc;
+  __syncthreads();
+  unsigned long long wbase = 0;
+  for (int w = 0; w < wave; ++w) wbase += sh[w];
+  *tot = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return wbase + inc - v;
+}
+
+__global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt, int64_t n,
+                                                     unsigned long long* __restrict__ partial) {
+  __shared__ unsigned long long sh[4];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
+  unsigned long long s = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = b0 + threadIdx.x * 4 + k;
+    if (i < n) s += cnt[i];
+  }
+  unsigned long long tot;
+  block_excl_scan256(s, sh, &tot);
+  if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// single block: partial[i] <- base + exclusive prefix of partial
+__global__ __launch_bounds__(256) void k_scan_top(unsigned long long* __restrict__ partial, int64_t nb,
+                                                  const unsigned long long* __restrict__ base_src) {
+  __shared__ unsigned long long sh[4];
+  unsigned long long carry = base_src ? *base_src : 0ull;
+  for (int64_t c = 0; c < nb; c += 256) {
+    const int64_t i = c + threadIdx.x;
+    const unsigned long long v = (i < nb) ? partial[i] : 0ull;
+    unsigned long long tot;
+    const unsigned long long ex = block_excl_scan256(v, sh, &tot);
+    if (i < nb) partial[i] = carry + ex;
+    carry += tot;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_scan_final(const uint32_t* __restrict__ cnt, int64_t n,
+                                                    const unsigned long long* __restrict__ partial,
+                                                    unsigned long long* __restrict__ off) {
+  __shared__ unsigned long long sh[4];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
+  uint32_t v[4];
+  unsigned long long s = 0;
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = b0 + threadIdx.x * 4 + k;
+    v[k] = (i < n) ? cnt[i] : 0u;
+    s += v[k];
+  }
+  unsigned long long tot;
+  unsigned long long ex = block_excl_scan256(s, sh, &tot) + partial[blockIdx.x];
+  for (int k = 0; k < 4; ++k) {
+    const int64_t i = b0 + threadIdx.x * 4 + k;
+    if (i < n) off[i] = ex;
+    ex += v[k];
+    if (i == n - 1) off[n] = ex;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// in-order compaction: one wave per tile, 4 tiles per block
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tile_write(const Job* __restrict__ jobs, int n_jobs,
+                                                    int64_t total_tiles,
+                                                    const unsigned long long* __restrict__ mask,
+                                                    const unsigned long long* __restrict__ off,
+                                                    uint32_t* __restrict__ list) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int64_t gt = (int64_t)blockIdx.x * 4 + wave;
+  if (gt >= total_tiles) return;
+  const int ji = find_job(n_jobs, gt, [&](int k) { return jobs[k].tile_base; });
+  const Job& jb = jobs[ji];
+  const int tile = (int)(gt - jb.tile_base);
+  const unsigned long long* mrow = mask + jb.mask_base + (int64_t)tile * jb.nw;
+  unsigned long long pos = off[gt];
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int c = 0; c < jb.nw; ++c) {
+    const unsigned long long word = mrow[c];
+    if ((word >> lane) & 1ull) list[pos + __popcll(word & lt)] = (uint32_t)(c * 64 + lane);
+    pos += __popcll(word);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// (c) blend.  One block per tile, one thread per pixel (TS*TS threads).
+//   PHASE 0: dense (all N), PHASE 1: culled first pass, PHASE 2: culled second pass+finish.
+// Per pixel state: M (running max of log2 logits), S = sum mask*2^(l-M), Z = sum
+// mask*2^(l-M)*t, E = sum exp(d2), Zall = sum nan_to_num(t), nan flag.
+// ------------------------------------------------------------------------------------------
+struct BlendArgs {
+  const Job* jobs;
+  int n_jobs;
+  const float* rec;
+  const unsigned long long* off1;
+  const unsigned long long* off2;
+  const uint32_t* list;
+  float* st;            // per-pixel state, 6 planes of total_pix: M S Z E Zall D
+  int64_t total_pix;
+  float* tile_lmin;
+  const float* tile_ubm;
+  Counters* cnt;
+  int save;
+};
+
+template <int TS, int GB, int PHASE>
+__global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
+  constexpr int NT = TS * TS;
+  constexpr int SUB = 8;
+  __shared__ float4 lds[GB * 4];
+  __shared__ float sh_red[NT / 64 + 1];
+
+  const int64_t gt = blockIdx.x;
+  const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
+  const Job& jb = A.jobs[ji];
+  const int tile = (int)(gt - jb.tile_base);
+  const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
+  const int px = tx * TS + (threadIdx.x % TS);
+  const int py = ty * TS + (threadIdx.x / TS);
+  const bool inimg = (px < jb.W) && (py < jb.H);
+  const int64_t pix = jb.pix_base + (int64_t)(inimg ? py : 0) * jb.W + (inimg ? px : 0);
+  const float xf = (float)px - jb.cx;
+  const float yf = (float)py - jb.cy;
+  const float b3 = jb.beta3;
+  const float nb2l = -jb.beta2 * kLog2e;
+  const float kd = -0.5f * kLog2e;
+
+  float M, S, Z, E, Zall;
+  if (PHASE == 2) {
+    M = A.st[pix];
+    S = A.st[A.total_pix + pix];
+    Z = A.st[2 * A.total_pix + pix];
+    E = A.st[3 * A.total_pix + pix];
+    Zall = A.st[4 * A.total_pix + pix];
+  } else {
+    M = -3.402823466e38f; S = 0.f; Z = 0.f; E = 0.f; Zall = 0.f;
+  }
+  bool nanf = false;
+
+  int64_t start, end;
+  if (PHASE == 0) { start = 0; end = jb.n; }
+  else if (PHASE == 1) { start = (int64_t)A.off1[gt]; end = (int64_t)A.off1[gt + 1]; }
+  else { start = (int64_t)A.off2[gt]; end = (int64_t)A.off2[gt + 1]; }
+  const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
+
+  for (int64_t base = start; base < end; base += GB) {
+    const int nb = (int)min<int64_t>(GB, end - base);
+    const int nbr = (nb + SUB - 1) / SUB * SUB;
+    for (int i = threadIdx.x; i < nbr * 4; i += NT) {
+      const int k = i >> 2, part = i & 3;
+      float4 v;
+      if (k < nb) {
+        const int64_t gid = (PHASE == 0) ? base + k : (int64_t)A.list[base + k];
+        v = rec4[gid * 4 + part];
+      } else {
+        // null record: u = (1,0,0), q = 0, w = -inf -> contributes nothing
+        v = (part == 1) ? make_float4(0.f, 0.f, 1.f, 0.f)
+            : (part == 3) ? make_float4(-INFINITY, -INFINITY, 0.f, 0.f)
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
+      lds[i] = v;
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < nbr; k0 += SUB) {
+      float ls[SUB], tsv[SUB];
+      float mx = M;
+#pragma unroll
+      for (int s = 0; s < SUB; ++s) {
+        const float4 r0 = lds[(k0 + s) * 4 + 0];
+        const float4 r1 = lds[(k0 + s) * 4 + 1];
+        const float4 r2 = lds[(k0 + s) * 4 + 2];
+        const float4 r3 = lds[(k0 + s) * 4 + 3];
+        // a = r0.xyz, b = (r0.w, r1.x, r1.y), g = (r1.z, r1.w, r2.x), q = (r2.y, r2.z, r2.w)
+        const float u0 = fmaf(r0.x, xf, fmaf(r0.w, yf, r1.z));
+        const float u1 = fmaf(r0.y, xf, fmaf(r1.x, yf, r1.w));
+        const float u2 = fmaf(r0.z, xf, fmaf(r1.y, yf, r2.x));
+        const float vsv = fmaf(u0, u0, fmaf(u1, u1, u2 * u2));
+        const float psv = fmaf(r2.y, u0, fmaf(r2.z, u1, r2.w * u2));
+        const float t = psv * __builtin_amdgcn_rcpf(vsv);
+        const float v0 = fmaf(t, u0, -r2.y);
+        const float v1 = fmaf(t, u1, -r2.z);
+        const float v2 = fmaf(t, u2, -r2.w);
+        const float d0 = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
+        const float d2l = fmaf(kd, d0, r3.y);           // log2(e) * d2
+        E += __builtin_amdgcn_exp2f(d2l);
+        const float l = fmaf(b3, d2l, nb2l * t);        // log2(e) * (b3 d2 - b2 t)
+        nanf = nanf || (l != l);
+        Zall += nan_to_num(t);
+        ls[s] = l;
+        tsv[s] = t;
+        mx = fmaxf(mx, l);
+      }
+      if (mx > M) {
+        const float sc = __builtin_amdgcn_exp2f(M - mx);
+        S *= sc;
+        Z *= sc;
+        M = mx;
+      }
+#pragma unroll
+      for (int s = 0; s < SUB; ++s) {
+        float sv = __builtin_amdgcn_exp2f(ls[s] - M);
+        sv = (tsv[s] > 0.0f) ? sv : 0.0f;
+        S += sv;
+        Z = fmaf(sv, tsv[s], Z);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (PHASE == 1) {
+    if (inimg) {
+      A.st[pix] = M;
+      A.st[A.total_pix + pix] = S;
+      A.st[2 * A.total_pix + pix] = Z;
+      A.st[3 * A.total_pix + pix] = E;
+      A.st[4 * A.total_pix + pix] = Zall;
+    }
+    // exact per-tile minimum of log2 sum exp of the unmasked listed logits; a pixel whose
+    // phase-1 listed logits are NaN gets -inf (its tile then lists everything in phase 2)
+    float lm = (S > 0.0f && !nanf) ? M + __log2f(S) : -INFINITY;
+    if (!inimg) lm = INFINITY;
+    lm = block_min<NT>(lm, sh_red);
+    if (threadIdx.x == 0) {
+      A.tile_lmin[gt] = lm;
+      atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
+                                   (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS)));
+    }
+    return;
+  }
+
+  // finish (PHASE 0 and 2)
+  float ncnt;
+  if (PHASE == 0) ncnt = (float)jb.n;
+  else ncnt = (float)((A.off1[gt + 1] - A.off1[gt]) + (A.off2[gt + 1] - A.off2[gt]));
+  float D;
+  if (ncnt == 0.0f) D = 0.0f;
+  else if (nanf) D = Zall / ncnt;
+  else D = (Z + 1e-20f * Zall) / (S + 1e-20f * ncnt);
+  const float alpha = -expm1f(-E);
+  if (inimg) {
+    jb.depth[pix - jb.pix_base] = D;
+    jb.alpha[pix - jb.pix_base] = alpha;
+    if (A.save) {
+      A.st[pix] = M;
+      A.st[A.total_pix + pix] = S;
+      A.st[2 * A.total_pix + pix] = Z;
+      A.st[3 * A.total_pix + pix] = E;
+      A.st[4 * A.total_pix + pix] = Zall;
+      A.st[5 * A.total_pix + pix] = D;
+    }
+  }
+  if (PHASE == 2) {
+    // weight floor: 1e-20 * n relative to S can only matter if a dropped pair that may lie
+    // behind the camera pushes the true max logit far above M
+    const float ubm = A.tile_ubm[gt];
+    const float lhs = -43.185f + __log2f((float)max(jb.n, 1)) + fmaxf(0.0f, ubm - M);
+    const bool fl = inimg && !(lhs < ((S > 0.0f) ? __log2f(S) : -INFINITY));
+    const int any = __syncthreads_or(fl ? 1 : 0);
+    if (threadIdx.x == 0) {
+      if (any) atomicAdd(&A.cnt->floor_tiles, 1ull);
+      atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
+                                   (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS)));
+    }
+  } else {
+    if (threadIdx.x == 0)
+      atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
+                                   (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS)));
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// launch helpers (called from the host code in fe_capi.cpp)
+// ------------------------------------------------------------------------------------------
+}  // namespace fe
+
+#include "fe_launch.h"
+
+namespace fe {
+
+hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs) {
+  const int blocks = max(1, min(64, (n_jobs + 255) / 256));
+  hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, s, cnt, job_flags, n_jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec,
+                             int32_t* job_flags) {
+  if (max_n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_preprocess, dim3((max_n + 255) / 256, n_jobs), dim3(256), 0, s, jobs, rec,
+                     job_flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_tiles,
+                            int ts, const float* rec, unsigned long long* mask, uint32_t* count,
+                            const float* tile_lmin, float* tile_ubm, const int32_t* job_flags) {
+  if (phase == 1)
+    hipLaunchKernelGGL(k_tile_mask<1>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts,
+                       rec, mask, count, tile_lmin, tile_ubm, job_flags);
+  else
+    hipLaunchKernelGGL(k_tile_mask<2>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts,
+                       rec, mask, count, tile_lmin, tile_ubm, job_flags);
+  return hipGetLastError();
+}
+
+int64_t scan_partial_count(int64_t n) { return (n + kScanBlock - 1) / kScanBlock; }
+
+hipError_t launch_scan(hipStream_t s, const uint32_t* cnt, int64_t n, unsigned long long* partial,
+                       unsigned long long* off, const unsigned long long* base_src) {
+  const int64_t nb = scan_partial_count(n);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, s, cnt, n, partial);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, partial, nb, base_src);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(256), 0, s, cnt, n, partial, off);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t total_tiles,
+                             const unsigned long long* mask, const unsigned long long* off,
+                             uint32_t* list) {
+  hipLaunchKernelGGL(k_tile_write, dim3((unsigned)((total_tiles + 3) / 4)), dim3(256), 0, s, jobs, n_jobs,
+                     total_tiles, mask, off, list);
+  return hipGetLastError();
+}
+
+template <int TS, int GB>
+static hipError_t launch_blend_tg(hipStream_t s, int phase, int64_t total_tiles, const BlendArgs& a) {
+  const dim3 grid((unsigned)total_tiles), block(TS * TS);
+  if (phase == 0) hipLaunchKernelGGL((k_blend<TS, GB, 0>), grid, block, 0, s, a);
+  else if (phase == 1) hipLaunchKernelGGL((k_blend<TS, GB, 1>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((k_blend<TS, GB, 2>), grid, block, 0, s, a);
+  return hipGetLastError();
+}
+
+template <int TS>
+static hipError_t launch_blend_t(hipStream_t s, int gb, int phase, int64_t total_tiles, const BlendArgs& a) {
+  switch (gb) {
+    case 32: return launch_blend_tg<TS, 32>(s, phase, total_tiles, a);
+    case 64: return launch_blend_tg<TS, 64>(s, phase, total_tiles, a);
+    case 128: return launch_blend_tg<TS, 128>(s, phase, total_tiles, a);
+    case 256: return launch_blend_tg<TS, 256>(s, phase, total_tiles, a);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_tiles, const Job* jobs,
+                        int n_jobs, const float* rec, const unsigned long long* off1,
+                        const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
+                        float* tile_lmin, const float* tile_ubm, Counters* cnt, int save) {
+  BlendArgs a{jobs, n_jobs, rec, off1, off2, list, st, total_pix, tile_lmin, tile_ubm, cnt, save};
+  switch (ts) {
+    case 8: return launch_blend_t<8>(s, gb, phase, total_tiles, a);
+    case 16: return launch_blend_t<16>(s, gb, phase, total_tiles, a);
+    case 32: return launch_blend_t<32>(s, gb, phase, total_tiles, a);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace fe

@@ -1,0 +1,155 @@
+/*
+ * fuzzy_ellipses_amd.h -- C ABI of the MI355X fuzzy-metaball rasterizer.
+ *
+ * Plain C: pointers, sizes and POD structs only.  Every entry point is asynchronous on the
+ * caller's HIP stream, performs no host synchronisation on the render path and returns a
+ * status code instead of exiting or throwing.
+ *
+ * Reference interfaces replaced (paths are relative to the reference repository):
+ *   - XLA legacy custom-call target "rasterize_gaussians_fwd",
+ *     void(cudaStream_t, void** buffers, const char* opaque, size_t)   rasterize_main.h:26-30,
+ *     registered from ext.cpp:22-32, descriptor rasterize_main.h:13-19.
+ *   - The math it is meant to reproduce: baseline/fm_render.py:11-126 (render_func_rays,
+ *     render_func_quat).
+ *   - Gradients: the reference has no backward kernel; callers used jax.grad of
+ *     baseline/fm_render.py (shape_from_silhouette.ipynb, "optim_bwd").
+ *
+ * Conventions (same as baseline/fm_render.py):
+ *   means       (N,3) float32 row-major, world coordinates.
+ *   prec        (N,3,3) float32 row-major; only the upper triangle is read (triu), the
+ *               density of Gaussian j at x is w_j - 0.5 |U_j^T (x - mu_j)|^2.
+ *   weights_log (N,)  float32 log-weights w_j.
+ *   camera      pinhole intrinsics + pose: the ray of pixel (x, y) is
+ *               r = ((x-cx)/fx, (y-cy)/fy, 1) @ R, starting at t; pixels row-major y*W+x.
+ *   outputs     depth (H,W), alpha (H,W) float32, exactly render_func_rays' outputs
+ *               (no nan_to_num; the reference binding applies it afterwards).
+ * All array pointers are device pointers; structs are read on the host during the call.
+ * `stream` is a hipStream_t passed as void* so this header needs no HIP include
+ * (NULL = the default stream).
+ */
+#ifndef FUZZY_ELLIPSES_AMD_H
+#define FUZZY_ELLIPSES_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum fe_status {
+  FE_OK = 0,
+  FE_ERR_INVALID_ARGUMENT = 1,  /* bad sizes, null pointers, unsupported tile size */
+  FE_ERR_WORKSPACE = 2,         /* workspace missing or smaller than fe_plan reported */
+  FE_ERR_HIP = 3,               /* a HIP runtime call failed (see fe_last_error_string) */
+  FE_ERR_NO_DEVICE = 4
+} fe_status;
+
+/* Pinhole camera.  R is row-major; its rows are the camera x, y, z axes in world
+ * coordinates (world ray = camera ray @ R, baseline/fm_render.py:116). */
+typedef struct fe_camera {
+  float R[9];
+  float t[3];
+  float fx, fy, cx, cy;
+  int32_t height, width;
+} fe_camera;
+
+/* One scene: N Gaussians in device memory. */
+typedef struct fe_scene {
+  int32_t n;
+  const float* means;        /* (N,3)   */
+  const float* prec;         /* (N,3,3) upper triangle read */
+  const float* weights_log;  /* (N,)    */
+} fe_scene;
+
+/* One view of a scene: camera, beta hyper-parameters and output images. */
+typedef struct fe_view {
+  int32_t scene;   /* index into the scene array */
+  fe_camera cam;
+  float beta2, beta3;
+  float* depth;    /* (H,W) device, written by fe_forward */
+  float* alpha;    /* (H,W) device, written by fe_forward */
+} fe_view;
+
+/* Gradient buffers of one scene (device, accumulated into: zero them first). */
+typedef struct fe_scene_grad {
+  float* d_means;        /* (N,3)   */
+  float* d_prec;         /* (N,3,3) only the upper triangle is written (lower stays as given) */
+  float* d_weights_log;  /* (N,)    */
+} fe_scene_grad;
+
+/* Upstream gradients of one view (device, (H,W) each; either may be NULL = zero). */
+typedef struct fe_view_cotangent {
+  const float* d_depth;
+  const float* d_alpha;
+} fe_view_cotangent;
+
+enum {
+  FE_MODE_CULLED = 0,  /* two-phase error-bounded tile culling (default) */
+  FE_MODE_DENSE = 1    /* every tile evaluates every Gaussian */
+};
+
+typedef struct fe_options {
+  int32_t tile;        /* tile edge in pixels: 8, 16 or 32 (0 = 16) */
+  int32_t batch;       /* LDS batch depth in Gaussians: 32, 64, 128 or 256 (0 = 64) */
+  int32_t mode;        /* FE_MODE_* */
+  int32_t save_for_backward; /* keep per-pixel aggregates for fe_backward */
+} fe_options;
+
+/* Diagnostics written by fe_forward into the workspace and readable with fe_read_stats. */
+typedef struct fe_stats {
+  int64_t pairs_phase1;      /* (tile, Gaussian) pairs listed by the alpha bound */
+  int64_t pairs_phase2;      /* pairs added by the depth bound */
+  int64_t evals;             /* ray-Gaussian evaluations performed (pixels x listed) */
+  int64_t floor_tiles;       /* tiles where the 1e-20 weight floor could matter (0 expected) */
+} fe_stats;
+
+/* ---- library ---- */
+int32_t fe_version(void);
+const char* fe_status_string(fe_status s);
+const char* fe_last_error_string(void);
+
+/* ---- planning: pure host arithmetic, no device access ----
+ * Reports the workspace bytes fe_forward/fe_backward need for this batch.  The workspace
+ * holds job descriptors, per-Gaussian records, tile masks and lists (sized for the worst
+ * case, so no host read-back is ever needed) and, with save_for_backward, per-pixel
+ * aggregates.  Replaces the per-call GpuMemoryPool of rasterizer_impl.cu:47-79. */
+fe_status fe_plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* views, int32_t n_views,
+                  const fe_options* opt, size_t* workspace_bytes);
+
+/* ---- forward ----
+ * Renders every view into its depth/alpha buffers.  Replaces RasterizeGaussiansCUDAJAX
+ * (rasterize_main.cu:29-114) and Rasterizer::forwardJAX (rasterizer_impl.cu:174-344). */
+fe_status fe_forward(void* stream, const fe_scene* scenes, int32_t n_scenes,
+                     const fe_view* views, int32_t n_views, const fe_options* opt,
+                     void* workspace, size_t workspace_bytes);
+
+/* ---- backward ----
+ * Given the workspace of the matching fe_forward (save_for_backward=1, untouched since),
+ * accumulates d(loss)/d(means, prec upper triangle, weights_log) for every scene.
+ * No reference counterpart: the reference took gradients from jax.grad of
+ * baseline/fm_render.py:11-109. */
+fe_status fe_backward(void* stream, const fe_scene* scenes, int32_t n_scenes,
+                      const fe_view* views, int32_t n_views, const fe_options* opt,
+                      const fe_view_cotangent* cot, const fe_scene_grad* grads,
+                      void* workspace, size_t workspace_bytes);
+
+/* Copies the diagnostics of the last fe_forward on this workspace to host (synchronises
+ * the stream; not for the hot path). */
+fe_status fe_read_stats(void* stream, const void* workspace, size_t workspace_bytes,
+                        fe_stats* out);
+
+/* ---- single-view convenience: the drop-in for the reference custom call ----
+ * Same arguments as render_func_quat with the rotation given as a matrix. */
+fe_status fe_render_workspace_bytes(int32_t n, int32_t height, int32_t width, int32_t tile,
+                                    size_t* bytes);
+fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float* prec,
+                        const float* weights_log, const fe_camera* cam, float beta2,
+                        float beta3, float* depth, float* alpha, void* workspace,
+                        size_t workspace_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FUZZY_ELLIPSES_AMD_H */

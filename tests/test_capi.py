@@ -1,0 +1,90 @@
+"""C ABI checks that need no GPU: the library loads, exports every declared entry point,
+and the host-only planner validates its arguments and sizes the workspace."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            names |= set(re.findall(r"^\s*(?:[\w\*]+\s+)+\**(fe_\w+)\s*\(", txt, re.M))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol(fea):
+    from fuzzy_ellipses_amd import _lib
+    lib = _lib.load()
+    declared = _declared()
+    assert len(declared) >= 9
+    missing = [n for n in declared if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.EXPORTS) <= set(declared)
+    assert lib.fe_version() >= 101
+
+
+def _one_view(fea, H=480, W=640, n=4096, tile=16, batch=64, mode=0, save=0):
+    from fuzzy_ellipses_amd import _lib
+    sc = (_lib.FeScene * 1)(_lib.FeScene(n, None, None, None))
+    cam = _lib.FeCamera()
+    cam.fx = cam.fy = 600.0
+    cam.cx, cam.cy = W / 2, H / 2
+    cam.height, cam.width = H, W
+    vw = (_lib.FeView * 1)(_lib.FeView(0, cam, 21.4, 2.66, None, None))
+    opt = _lib.FeOptions(tile, batch, mode, save)
+    nbytes = ctypes.c_size_t(0)
+    st = _lib.load().fe_plan(sc, 1, vw, 1, ctypes.byref(opt), ctypes.byref(nbytes))
+    return st, nbytes.value
+
+
+def test_plan_sizes_workspace_for_worst_case(fea):
+    st, nbytes = _one_view(fea)
+    assert st == 0
+    tiles = 30 * 40
+    # lists sized for every (tile, Gaussian) pair; records 64 B per Gaussian
+    assert nbytes >= tiles * 4096 * 4 + 4096 * 64
+    st_d, nb_dense = _one_view(fea, mode=1)
+    assert st_d == 0 and nb_dense < nbytes
+    st_s, nb_save = _one_view(fea, save=1)
+    assert st_s == 0 and nb_save > nbytes
+
+
+@pytest.mark.parametrize("kw", [dict(tile=12), dict(batch=48), dict(mode=7), dict(H=0), dict(n=-1)])
+def test_plan_rejects_bad_arguments(fea, kw):
+    from fuzzy_ellipses_amd import _lib
+    st, _ = _one_view(fea, **kw)
+    assert st == 1
+    assert _lib.load().fe_last_error_string().decode().startswith("fe_plan")
+
+
+def test_plan_zero_gaussians_ok(fea):
+    st, nbytes = _one_view(fea, n=0, H=8, W=8)
+    assert st == 0 and nbytes > 0
+
+
+def test_forward_rejects_small_workspace(fea):
+    """fe_forward validates the workspace before touching the device."""
+    from fuzzy_ellipses_amd import _lib
+    lib = _lib.load()
+    sc = (_lib.FeScene * 1)(_lib.FeScene(4, None, None, None))
+    cam = _lib.FeCamera()
+    cam.fx = cam.fy = 8.0
+    cam.height = cam.width = 8
+    vw = (_lib.FeView * 1)(_lib.FeView(0, cam, 21.4, 2.66, None, None))
+    opt = _lib.FeOptions(8, 32, 0, 0)
+    st = lib.fe_forward(None, sc, 1, vw, 1, ctypes.byref(opt), None, 0)
+    assert st == 2  # FE_ERR_WORKSPACE
+
+
+def test_python_api_fails_loudly_without_library(fea, monkeypatch, tmp_path):
+    from fuzzy_ellipses_amd import _lib
+    monkeypatch.setattr(_lib, "_lib", None)
+    with pytest.raises(_lib.FeError):
+        _lib.load(str(tmp_path / "missing.so"))
