@@ -141,7 +141,24 @@ class Renderer:
         _lib.check(self.lib.fe_read_stats(_stream(self.device), self._ws.data_ptr(), self._ws.numel(),
                                           ctypes.byref(st)), "fe_read_stats")
         return {"pairs_phase1": st.pairs_phase1, "pairs_phase2": st.pairs_phase2, "evals": st.evals,
-                "floor_tiles": st.floor_tiles}
+                "evals_phase1": st.evals_phase1, "floor_tiles": st.floor_tiles,
+                "floor_unresolved": st.floor_unresolved}
+
+    def debug_state(self) -> dict:
+        """Host copies of the last forward's workspace internals (tests only): per-Gaussian
+        records, tile ranges and lists of both phases, per-tile phase-1 thresholds."""
+        torch.cuda.synchronize(self.device)
+        ws = self._ws.cpu().numpy()
+        h = ws[:8 * 12].view(np.uint64)
+        magic, o_cnt, o_off1, o_off2, T, mode, o_rec, o_list, o_lmin, n_rec, o_st, n_pix = [int(x) for x in h]
+        out = {"records": ws[o_rec:o_rec + 64 * n_rec].view(np.float32).reshape(n_rec, 16), "mode": mode}
+        if mode == _lib.FE_MODE_CULLED:
+            off1 = ws[o_off1:o_off1 + 8 * (T + 1)].view(np.uint64).astype(np.int64)
+            off2 = ws[o_off2:o_off2 + 8 * (T + 1)].view(np.uint64).astype(np.int64)
+            total = int(off2[-1])
+            out.update(off1=off1, off2=off2, list=ws[o_list:o_list + 4 * total].view(np.uint32).copy(),
+                       tile_lmin=ws[o_lmin:o_lmin + 4 * T].view(np.float32).copy())
+        return out
 
     # -- backward -----------------------------------------------------------------------
     def backward(self, d_depth: Sequence[Optional[torch.Tensor]], d_alpha: Sequence[Optional[torch.Tensor]]):

@@ -17,8 +17,11 @@ FE_MODE_DENSE = 1
 
 EXPORTS = [
     "fe_version", "fe_status_string", "fe_last_error_string", "fe_plan", "fe_forward",
-    "fe_backward", "fe_read_stats", "fe_render_workspace_bytes", "fe_render_fwd",
+    "fe_backward", "fe_read_stats", "fe_render_workspace_bytes", "fe_render_fwd", "fe_profile_enable",
+    "fe_profile_read",
 ]
+KERNELS = ["preprocess", "mask1", "scan1", "write1", "blend1", "mask2", "scan2", "write2", "blend2",
+           "blend_dense", "bwd_blend", "bwd_finalize"]
 
 
 class FeCamera(ctypes.Structure):
@@ -53,7 +56,12 @@ class FeOptions(ctypes.Structure):
 
 class FeStats(ctypes.Structure):
     _fields_ = [("pairs_phase1", ctypes.c_int64), ("pairs_phase2", ctypes.c_int64),
-                ("evals", ctypes.c_int64), ("floor_tiles", ctypes.c_int64)]
+                ("evals", ctypes.c_int64), ("evals_phase1", ctypes.c_int64), ("floor_tiles", ctypes.c_int64),
+                ("floor_unresolved", ctypes.c_int64)]
+
+
+class FeKernelTimes(ctypes.Structure):
+    _fields_ = [("ms", ctypes.c_double * 12), ("launches", ctypes.c_int64 * 12)]
 
 
 class FeError(RuntimeError):
@@ -97,8 +105,23 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fe_render_fwd.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
                                   ctypes.c_void_p, ctypes.POINTER(FeCamera), ctypes.c_float, ctypes.c_float,
                                   ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.fe_profile_enable.restype = ctypes.c_int
+    lib.fe_profile_enable.argtypes = [ctypes.c_int32]
+    lib.fe_profile_read.restype = ctypes.c_int
+    lib.fe_profile_read.argtypes = [ctypes.POINTER(FeKernelTimes), ctypes.c_int32]
     _lib = lib
     return lib
+
+
+def profile_enable(on: bool) -> None:
+    check(load().fe_profile_enable(1 if on else 0), "fe_profile_enable")
+
+
+def profile_read(reset: bool = True) -> dict:
+    """{kernel: (total_ms, launches)} of the event-timed launches since the last reset."""
+    kt = FeKernelTimes()
+    check(load().fe_profile_read(ctypes.byref(kt), 1 if reset else 0), "fe_profile_read")
+    return {k: (kt.ms[i], kt.launches[i]) for i, k in enumerate(KERNELS)}
 
 
 def check(status: int, what: str = "") -> None:

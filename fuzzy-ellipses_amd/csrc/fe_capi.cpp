@@ -43,15 +43,16 @@ struct Layout {
   int max_n = 0;
   int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0;
   int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
-  size_t o_header = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_mask = 0, o_count = 0,
+  size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
-         o_gacc = 0, bytes = 0;
+         o_gacc = 0, o_bjobs = 0, bytes = 0;
 };
 
 // A small header at the start of the workspace lets fe_read_stats find the counters.
 struct WsHeader {
   uint64_t magic;
   uint64_t o_counters, o_off1, o_off2, total_tiles, mode;
+  uint64_t o_rec, o_list, o_lmin, total_rec, o_state, total_pix;
 };
 constexpr uint64_t kMagic = 0x4645414d44313031ull;  // "FEAMD101"
 
@@ -120,6 +121,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       jb.c_alpha = (float)(ln_n + 14.0);
       // depth: dropped softmax weight fraction <= n * exp(-delta) = exp(-20) ~ 2e-9
       jb.delta2 = (float)((ln_n + 20.0) * 1.4426950408889634);
+      jb.floor_thr = (float)(13.0 * 3.321928094887362 - ln_n * 1.4426950408889634);
       jb.n = sc.n; jb.H = c.height; jb.W = c.width;
       jb.tiles_x = tx; jb.tiles_y = ty; jb.n_tiles = tx * ty;
       jb.nw = nw;
@@ -157,6 +159,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->o_state = p; p = align256(p + sizeof(float) * 6 * pix);
   if (o.mode == FE_MODE_CULLED) {
     L->o_mask = p; p = align256(p + sizeof(unsigned long long) * std::max<int64_t>(maskw, 1));
+    L->o_mask2 = p; p = align256(p + sizeof(unsigned long long) * std::max<int64_t>(maskw, 1));
     L->o_count = p; p = align256(p + sizeof(uint32_t) * tiles);
     L->o_off1 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
     L->o_off2 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
@@ -165,6 +168,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   }
   if (L->save) {
     L->o_gacc = p; p = align256(p + sizeof(float) * 16 * (size_t)std::max<int64_t>(rec, 1));
+    L->o_bjobs = p; p = align256(p + sizeof(BwdJob) * n_views);
   }
   L->bytes = p;
   return FE_OK;
@@ -205,6 +209,45 @@ static fe_status upload(hipStream_t s, void* dst, const void* src, size_t bytes)
   return FE_OK;
 }
 
+// Optional per-kernel event timing.
+struct Profiler {
+  std::mutex mu;
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  struct Rec { int k; hipEvent_t a, b; };
+  std::vector<Rec> pending;
+  double ms[FE_K_COUNT] = {};
+  int64_t launches[FE_K_COUNT] = {};
+  hipEvent_t get() {
+    if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+};
+static Profiler g_prof;
+
+// Runs `launch` between two recorded events when profiling is on.
+template <typename F>
+static hipError_t timed(hipStream_t s, int k, F launch) {
+  if (!g_prof.on) return launch();
+  hipEvent_t a, b;
+  {
+    std::lock_guard<std::mutex> lk(g_prof.mu);
+    a = g_prof.get();
+    b = g_prof.get();
+  }
+  if (!a || !b) return launch();
+  hipError_t e = hipEventRecord(a, s);
+  if (e != hipSuccess) return e;
+  e = launch();
+  if (e != hipSuccess) return e;
+  e = hipEventRecord(b, s);
+  std::lock_guard<std::mutex> lk(g_prof.mu);
+  g_prof.pending.push_back({k, a, b});
+  return e;
+}
+
 template <typename T>
 static T* at(void* ws, size_t off) {
   return reinterpret_cast<T*>(reinterpret_cast<char*>(ws) + off);
@@ -233,7 +276,8 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   }
   // header + jobs in one upload
   std::vector<char> blob(L.o_counters);
-  WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode};
+  WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode,
+             L.o_rec, L.o_list, L.o_lmin, (uint64_t)L.total_rec, L.o_state, (uint64_t)L.total_pix};
   memcpy(blob.data() + L.o_header, &h, sizeof(h));
   memcpy(blob.data() + L.o_jobs, jobs.data(), sizeof(Job) * n_views);
   st = upload(s, ws, blob.data(), blob.size());
@@ -247,36 +291,103 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   float* lmin = at<float>(ws, L.o_lmin);
   float* ubm = at<float>(ws, L.o_ubm);
   FE_HIP(launch_init(s, cnt, flags, L.n_jobs));
-  FE_HIP(launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, flags));
+  FE_HIP(timed(s, FE_K_PREPROCESS, [&] { return launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, flags); }));
   if (L.mode == FE_MODE_DENSE) {
-    FE_HIP(launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, nullptr, nullptr, nullptr,
-                        state, L.total_pix, lmin, ubm, cnt, L.save));
+    FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
+      return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, nullptr, nullptr, nullptr, state,
+                          L.total_pix, lmin, ubm, cnt, L.save);
+    }));
     return FE_OK;
   }
   unsigned long long* mask = at<unsigned long long>(ws, L.o_mask);
+  unsigned long long* mask2 = at<unsigned long long>(ws, L.o_mask2);
   uint32_t* count = at<uint32_t>(ws, L.o_count);
   unsigned long long* off1 = at<unsigned long long>(ws, L.o_off1);
   unsigned long long* off2 = at<unsigned long long>(ws, L.o_off2);
   unsigned long long* partial = at<unsigned long long>(ws, L.o_partial);
   uint32_t* list = at<uint32_t>(ws, L.o_list);
   // phase 1
-  FE_HIP(launch_tile_mask(s, 1, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, count, lmin, ubm, flags));
-  FE_HIP(launch_scan(s, count, L.total_tiles, partial, off1, nullptr));
-  FE_HIP(launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list));
-  FE_HIP(launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state,
-                      L.total_pix, lmin, ubm, cnt, L.save));
+  FE_HIP(timed(s, FE_K_MASK1, [&] {
+    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
+                            cnt);
+  }));
+  FE_HIP(timed(s, FE_K_SCAN1, [&] { return launch_scan(s, count, L.total_tiles, partial, off1, nullptr); }));
+  FE_HIP(timed(s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }));
+  FE_HIP(timed(s, FE_K_BLEND1, [&] {
+    return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state, L.total_pix,
+                        lmin, ubm, cnt, L.save);
+  }));
   // phase 2
-  FE_HIP(launch_tile_mask(s, 2, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, count, lmin, ubm, flags));
-  FE_HIP(launch_scan(s, count, L.total_tiles, partial, off2, off1 + L.total_tiles));
-  FE_HIP(launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off2, list));
-  FE_HIP(launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state,
-                      L.total_pix, lmin, ubm, cnt, L.save));
+  FE_HIP(timed(s, FE_K_MASK2, [&] {
+    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
+                            cnt);
+  }));
+  FE_HIP(timed(s, FE_K_SCAN2, [&] {
+    return launch_scan(s, count, L.total_tiles, partial, off2, off1 + L.total_tiles);
+  }));
+  FE_HIP(timed(s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }));
+  FE_HIP(timed(s, FE_K_BLEND2, [&] {
+    return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state, L.total_pix,
+                        lmin, ubm, cnt, L.save);
+  }));
+  return FE_OK;
+}
+
+fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
+                   int32_t n_views, const fe_options* opt, const fe_view_cotangent* cot,
+                   const fe_scene_grad* grads, void* ws, size_t ws_bytes) {
+  fe_options o = {16, 64, FE_MODE_CULLED, 1};
+  if (opt) o = *opt;
+  o.save_for_backward = 1;
+  Layout L;
+  std::vector<Job> jobs;
+  fe_status st = plan(scenes, n_scenes, views, n_views, &o, &L, &jobs);
+  if (st != FE_OK) return st;
+  if (!cot || !grads) {
+    set_msg("fe_backward: null cotangent or gradient array");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  if (!ws || ws_bytes < L.bytes) {
+    set_msg("fe_backward: workspace missing or smaller than the forward's (save_for_backward=1)");
+    return FE_ERR_WORKSPACE;
+  }
+  std::vector<BwdJob> bj(n_views);
+  for (int v = 0; v < n_views; ++v) {
+    const fe_scene_grad& g = grads[views[v].scene];
+    bj[v].gD = cot[v].d_depth;
+    bj[v].gA = cot[v].d_alpha;
+    bj[v].dmeans = g.d_means;
+    bj[v].dprec = g.d_prec;
+    bj[v].dw = g.d_weights_log;
+  }
+  st = upload(s, at<char>(ws, L.o_bjobs), bj.data(), sizeof(BwdJob) * n_views);
+  if (st != FE_OK) return st;
+  const Job* dj = at<Job>(ws, L.o_jobs);
+  const void* dbj = at<char>(ws, L.o_bjobs);
+  float* gacc = at<float>(ws, L.o_gacc);
+  FE_HIP(launch_zero(s, gacc, 16 * L.total_rec));
+  const bool dense = (L.mode == FE_MODE_DENSE);
+  FE_HIP(timed(s, FE_K_BWD_BLEND, [&] {
+    return launch_bwd_blend(s, L.ts, L.gb, L.total_tiles, dj, dbj, L.n_jobs, at<float>(ws, L.o_rec),
+                            dense ? nullptr : at<unsigned long long>(ws, L.o_off1),
+                            dense ? nullptr : at<unsigned long long>(ws, L.o_off2),
+                            dense ? nullptr : at<uint32_t>(ws, L.o_list), at<float>(ws, L.o_state), L.total_pix,
+                            gacc, dense ? 1 : 0);
+  }));
+  FE_HIP(timed(s, FE_K_BWD_FINALIZE, [&] { return launch_bwd_finalize(s, dj, dbj, L.n_jobs, L.max_n, gacc); }));
   return FE_OK;
 }
 
 }  // namespace fe
 
 extern "C" {
+
+fe_status fe_backward(void* stream, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
+                      int32_t n_views, const fe_options* opt, const fe_view_cotangent* cot,
+                      const fe_scene_grad* grads, void* workspace, size_t workspace_bytes) {
+  return fe::backward(reinterpret_cast<hipStream_t>(stream), scenes, n_scenes, views, n_views, opt, cot, grads,
+                      workspace, workspace_bytes);
+}
 
 int32_t fe_version(void) { return 101; }
 
@@ -331,6 +442,38 @@ fe_status fe_read_stats(void* stream, const void* workspace, size_t workspace_by
   out->pairs_phase2 = (int64_t)(t2 - t1);
   out->evals = (int64_t)c.evals;
   out->floor_tiles = (int64_t)c.floor_tiles;
+  out->evals_phase1 = (int64_t)c.evals1;
+  out->floor_unresolved = (int64_t)c.floor_unresolved;
+  return FE_OK;
+}
+
+fe_status fe_profile_enable(int32_t on) {
+  std::lock_guard<std::mutex> lk(fe::g_prof.mu);
+  fe::g_prof.on = on != 0;
+  return FE_OK;
+}
+
+fe_status fe_profile_read(fe_kernel_times* out, int32_t reset) {
+  if (!out) return FE_ERR_INVALID_ARGUMENT;
+  std::lock_guard<std::mutex> lk(fe::g_prof.mu);
+  for (auto& r : fe::g_prof.pending) {
+    FE_HIP(hipEventSynchronize(r.b));
+    float ms = 0.0f;
+    FE_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+    fe::g_prof.ms[r.k] += ms;
+    fe::g_prof.launches[r.k] += 1;
+    fe::g_prof.pool.push_back(r.a);
+    fe::g_prof.pool.push_back(r.b);
+  }
+  fe::g_prof.pending.clear();
+  for (int k = 0; k < FE_K_COUNT; ++k) {
+    out->ms[k] = fe::g_prof.ms[k];
+    out->launches[k] = fe::g_prof.launches[k];
+    if (reset) {
+      fe::g_prof.ms[k] = 0.0;
+      fe::g_prof.launches[k] = 0;
+    }
+  }
   return FE_OK;
 }
 

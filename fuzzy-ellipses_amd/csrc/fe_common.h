@@ -36,6 +36,8 @@ struct Job {
   float beta2, beta3;
   float c_alpha;   // alpha cull constant, natural-log units: drop if d2 < -c_alpha on a tile
   float delta2;    // depth cull margin, log2 units
+  float floor_thr; // log2(1e13 / n): a dropped logit this far above the tile minimum makes the
+                   // 1e-20 weight floor visible, so the tile is completed densely
   int32_t n, H, W;
   int32_t tiles_x, tiles_y, n_tiles;
   int32_t nw;      // 64-bit mask words per tile = ceil(n / 64)
@@ -46,11 +48,22 @@ struct Job {
   int64_t pix_base;   // first pixel of this view in the per-pixel state arrays
 };
 
+// Per-view backward buffers: upstream gradients of the images and the scene's gradients.
+struct BwdJob {
+  const float* gD;
+  const float* gA;
+  float* dmeans;
+  float* dprec;
+  float* dw;
+};
+
 // Device counters (zeroed by k_init each forward).
 struct Counters {
   unsigned long long evals;
-  unsigned long long floor_tiles;
-  unsigned long long pad[6];
+  unsigned long long floor_tiles;       // tiles completed densely because the floor could matter
+  unsigned long long evals1;
+  unsigned long long floor_unresolved;  // pixels' tiles where it still could (must stay 0)
+  unsigned long long pad[4];
 };
 
 __device__ __forceinline__ float dot3(const float* x, const float* y) {
@@ -149,7 +162,8 @@ __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& t
   const float un = sqrtf(uu);
   pb.plus = (qu > 0.0f) && (pb.sd_hi < qn);
   pb.t_lb = pb.plus ? sqrtf(fmaxf(fmaf(-pb.sd_hi, pb.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
-  pb.t_any_lb = (un > rho) ? -(qn / (un - rho)) : -INFINITY;
+  // |q.u|/|u| <= sqrt(qq - d0_lo) on the tile, |u| >= |u_c| - rho
+  pb.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-pb.sd_lo, pb.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
   if (!(pb.sd_lo == pb.sd_lo)) pb.unbounded = true;  // NaN anywhere -> keep
   return pb;
 }

@@ -104,19 +104,24 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
 
 // ------------------------------------------------------------------------------------------
 // (b) tile membership: one block (4 waves) per tile, each wave tests 64 Gaussians at a time.
-// PHASE 1 keeps pairs whose density bound reaches exp(-c_alpha):  d0_lo/2 - w <= c_alpha.
-// PHASE 2 keeps pairs not kept before whose depth-logit bound reaches the tile threshold
-// tau = min_pixels(log2-sum-exp of phase-1 logits) - delta2, and records the largest logit
-// bound of dropped pairs that might lie behind the camera (for the weight-floor check).
+// PHASE 1 keeps pairs whose density bound reaches exp(-c_alpha):  d0_lo/2 - w <= c_alpha,
+//   writing mask1.
+// PHASE 2 keeps pairs absent from mask1 whose depth-logit bound reaches the tile threshold
+//   tau = min_pixels(log2-sum-exp of phase-1 logits) - delta2, writing mask2.  It also takes
+//   the largest logit bound `ubm` of the dropped pairs that might lie behind the camera: if
+//   such a pair could raise the max logit so far that the 1e-20 weight floor becomes visible
+//   (ubm - lmin >= log2(1e13 / n)), the tile is completed densely (every remaining pair).
 // ------------------------------------------------------------------------------------------
 template <int PHASE>
 __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs, int n_jobs, int ts,
                                                    const float* __restrict__ rec,
-                                                   unsigned long long* __restrict__ mask,
+                                                   unsigned long long* __restrict__ mask1,
+                                                   unsigned long long* __restrict__ mask2,
                                                    uint32_t* __restrict__ count,
                                                    const float* __restrict__ tile_lmin,
                                                    float* __restrict__ tile_ubm,
-                                                   const int32_t* __restrict__ job_flags) {
+                                                   const int32_t* __restrict__ job_flags,
+                                                   Counters* __restrict__ cnt_out) {
   __shared__ uint32_t sh_cnt[4];
   __shared__ float sh_ubm[4];
   const int64_t gt = blockIdx.x;
@@ -127,14 +132,14 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
   const bool dense_job = (job_flags[ji] & 1) != 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
-  const float tau = (PHASE == 2) ? tile_lmin[gt] - jb.delta2 : 0.0f;
+  const float lmin = (PHASE == 2) ? tile_lmin[gt] : 0.0f;
+  const float tau = lmin - jb.delta2;
   uint32_t cnt = 0;
   float ubm = -INFINITY;
-  unsigned long long* mrow = mask + jb.mask_base + (int64_t)tile * jb.nw;
+  const int64_t row = jb.mask_base + (int64_t)tile * jb.nw;
   for (int c = wave; c < jb.nw; c += 4) {
     const int j = c * 64 + lane;
-    unsigned long long old = 0ull;
-    if (PHASE == 2) old = mrow[c];
+    const unsigned long long old = (PHASE == 2) ? mask1[row + c] : 0ull;
     bool keep = false;
     const bool listed = (PHASE == 2) && ((old >> lane) & 1ull);
     if (j < jb.n && !listed) {
@@ -162,7 +167,10 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
       }
     }
     const unsigned long long word = __ballot(keep);
-    if (lane == 0) mrow[c] = word;
+    if (lane == 0) {
+      if (PHASE == 1) mask1[row + c] = word;
+      else mask2[row + c] = word;
+    }
     cnt += (uint32_t)__popcll(word);
   }
   if (PHASE == 2) {
@@ -170,10 +178,29 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
   }
   if (lane == 0) { sh_cnt[wave] = cnt; sh_ubm[wave] = ubm; }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    count[gt] = sh_cnt[0] + sh_cnt[1] + sh_cnt[2] + sh_cnt[3];
-    if (PHASE == 2) tile_ubm[gt] = fmaxf(fmaxf(sh_ubm[0], sh_ubm[1]), fmaxf(sh_ubm[2], sh_ubm[3]));
+  uint32_t total = sh_cnt[0] + sh_cnt[1] + sh_cnt[2] + sh_cnt[3];
+  if (PHASE == 2) {
+    const float ubm_t = fmaxf(fmaxf(sh_ubm[0], sh_ubm[1]), fmaxf(sh_ubm[2], sh_ubm[3]));
+    const bool dense_tile = (ubm_t - lmin >= jb.floor_thr);   // false when no pair is in doubt (-inf)
+    if (dense_tile) {
+      // complete the tile: every pair not in phase 1
+      __syncthreads();
+      uint32_t c2 = 0;
+      for (int c = wave; c < jb.nw; c += 4) {
+        const int j = c * 64 + lane;
+        const bool keep = (j < jb.n) && !((mask1[row + c] >> lane) & 1ull);
+        const unsigned long long word = __ballot(keep);
+        if (lane == 0) mask2[row + c] = word;
+        c2 += (uint32_t)__popcll(word);
+      }
+      if (lane == 0) sh_cnt[wave] = c2;
+      __syncthreads();
+      total = sh_cnt[0] + sh_cnt[1] + sh_cnt[2] + sh_cnt[3];
+      if (threadIdx.x == 0) atomicAdd(&cnt_out->floor_tiles, 1ull);
+    }
+    if (threadIdx.x == 0) tile_ubm[gt] = dense_tile ? -INFINITY : ubm_t;
   }
+  if (threadIdx.x == 0) count[gt] = total;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -421,8 +448,10 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
     lm = block_min<NT>(lm, sh_red);
     if (threadIdx.x == 0) {
       A.tile_lmin[gt] = lm;
-      atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
-                                   (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS)));
+      const unsigned long long ev = (unsigned long long)(end - start) *
+                                    (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS));
+      atomicAdd(&A.cnt->evals, ev);
+      atomicAdd(&A.cnt->evals1, ev);
     }
     return;
   }
@@ -456,14 +485,17 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
     const bool fl = inimg && !(lhs < ((S > 0.0f) ? __log2f(S) : -INFINITY));
     const int any = __syncthreads_or(fl ? 1 : 0);
     if (threadIdx.x == 0) {
-      if (any) atomicAdd(&A.cnt->floor_tiles, 1ull);
+      if (any) atomicAdd(&A.cnt->floor_unresolved, 1ull);
       atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
                                    (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS)));
     }
   } else {
-    if (threadIdx.x == 0)
-      atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
-                                   (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS)));
+    if (threadIdx.x == 0) {
+      const unsigned long long ev = (unsigned long long)(end - start) *
+                                    (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS));
+      atomicAdd(&A.cnt->evals, ev);
+      atomicAdd(&A.cnt->evals1, ev);
+    }
   }
 }
 
@@ -491,14 +523,15 @@ hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max
 }
 
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_tiles,
-                            int ts, const float* rec, unsigned long long* mask, uint32_t* count,
-                            const float* tile_lmin, float* tile_ubm, const int32_t* job_flags) {
+                            int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
+                            uint32_t* count, const float* tile_lmin, float* tile_ubm, const int32_t* job_flags,
+                            Counters* cnt) {
   if (phase == 1)
-    hipLaunchKernelGGL(k_tile_mask<1>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts,
-                       rec, mask, count, tile_lmin, tile_ubm, job_flags);
+    hipLaunchKernelGGL(k_tile_mask<1>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
+                       mask2, count, tile_lmin, tile_ubm, job_flags, cnt);
   else
-    hipLaunchKernelGGL(k_tile_mask<2>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts,
-                       rec, mask, count, tile_lmin, tile_ubm, job_flags);
+    hipLaunchKernelGGL(k_tile_mask<2>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
+                       mask2, count, tile_lmin, tile_ubm, job_flags, cnt);
   return hipGetLastError();
 }
 

@@ -101,7 +101,9 @@ typedef struct fe_stats {
   int64_t pairs_phase1;      /* (tile, Gaussian) pairs listed by the alpha bound */
   int64_t pairs_phase2;      /* pairs added by the depth bound */
   int64_t evals;             /* ray-Gaussian evaluations performed (pixels x listed) */
-  int64_t floor_tiles;       /* tiles where the 1e-20 weight floor could matter (0 expected) */
+  int64_t evals_phase1;      /* of which in the phase-1 (or dense) blend */
+  int64_t floor_tiles;       /* tiles completed densely because the 1e-20 weight floor could matter */
+  int64_t floor_unresolved;  /* tiles where it still could after phase 2 (always 0; a self-check) */
 } fe_stats;
 
 /* ---- library ---- */
@@ -138,6 +140,21 @@ fe_status fe_backward(void* stream, const fe_scene* scenes, int32_t n_scenes,
  * the stream; not for the hot path). */
 fe_status fe_read_stats(void* stream, const void* workspace, size_t workspace_bytes,
                         fe_stats* out);
+
+/* ---- kernel timing (diagnostics) ----
+ * When enabled, every kernel launch of fe_forward/fe_backward is bracketed by hipEvents on
+ * the caller's stream; fe_profile_read synchronises those events and reports the summed
+ * durations and launch counts per kernel, then optionally resets them. */
+enum {
+  FE_K_PREPROCESS = 0, FE_K_MASK1, FE_K_SCAN1, FE_K_WRITE1, FE_K_BLEND1, FE_K_MASK2, FE_K_SCAN2,
+  FE_K_WRITE2, FE_K_BLEND2, FE_K_BLEND_DENSE, FE_K_BWD_BLEND, FE_K_BWD_FINALIZE, FE_K_COUNT
+};
+typedef struct fe_kernel_times {
+  double ms[FE_K_COUNT];
+  int64_t launches[FE_K_COUNT];
+} fe_kernel_times;
+fe_status fe_profile_enable(int32_t on);
+fe_status fe_profile_read(fe_kernel_times* out, int32_t reset);
 
 /* ---- single-view convenience: the drop-in for the reference custom call ----
  * Same arguments as render_func_quat with the rotation given as a matrix. */

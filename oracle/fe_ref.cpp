@@ -265,7 +265,7 @@ static Bound bound_of(const float* r, float Xc, float Yc, float hx, float hy) {
   const float un = sqrtf(uu);
   o.plus = (qu > 0.0f) && (o.sd_hi < qn);
   o.t_lb = o.plus ? sqrtf(fmaxf(fmaf(-o.sd_hi, o.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
-  o.t_any_lb = (un > rho) ? -(qn / (un - rho)) : -INFINITY;
+  o.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-o.sd_lo, o.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
   if (!(o.sd_lo == o.sd_lo)) o.unbounded = true;
   return o;
 }
@@ -274,8 +274,8 @@ static Bound bound_of(const float* r, float Xc, float Yc, float hx, float hy) {
 // phase 1 whose depth-logit bound reaches tile_lmin[t] - delta2.  Writes counts[T] and the
 // concatenated, per-tile ascending Gaussian ids to `list` (capacity n*T); returns the total.
 int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, int ts, float cx, float cy,
-                     float beta2, float beta3, float c_alpha, float delta2, const float* tile_lmin,
-                     uint32_t* counts, uint32_t* list) {
+                     float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
+                     const float* tile_lmin, uint32_t* counts, uint32_t* list) {
   const int tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
   const float b3l = beta3 * kLog2e, b2l = beta2 * kLog2e;
   int64_t pos = 0;
@@ -288,6 +288,9 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
     const float hx = (float)(xl - x0) * 0.5f;
     const float hy = (float)(yl - y0) * 0.5f;
     uint32_t c = 0;
+    const int64_t start = pos;
+    float ubm = -INFINITY;
+    std::vector<char> in1(n > 0 ? n : 1, 0);
     for (int j = 0; j < n; ++j) {
       const float* r = rec + 16 * (size_t)j;
       int32_t fl;
@@ -295,6 +298,7 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
       const bool deg = (fl & 1) != 0;
       const Bound o = bound_of(r, Xc, Yc, hx, hy);
       const bool k1 = any_degenerate || deg || o.unbounded || (fmaf(0.5f * o.sd_lo, o.sd_lo, -r[12]) <= c_alpha);
+      in1[j] = k1;
       bool keep;
       if (!tile_lmin) {
         keep = k1;
@@ -303,11 +307,22 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
         const float dlo = fmaf(-0.5f * o.sd_lo, o.sd_lo, r[12]);
         const float ub = fmaf(b3l, dlo, -(b2l * o.t_lb));
         keep = !k1 && (o.unbounded || ub >= tau);
+        if (!k1 && !keep && !o.plus) ubm = fmaxf(ubm, fmaf(b3l, dlo, -(b2l * o.t_any_lb)));
       }
       if (keep) {
         list[pos++] = (uint32_t)j;
         ++c;
       }
+    }
+    if (tile_lmin && (ubm - tile_lmin[tile] >= floor_thr)) {
+      // the weight floor could matter: complete the tile with every pair not in phase 1
+      pos = start;
+      c = 0;
+      for (int j = 0; j < n; ++j)
+        if (!in1[j]) {
+          list[pos++] = (uint32_t)j;
+          ++c;
+        }
     }
     counts[tile] = c;
   }

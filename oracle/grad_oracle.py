@@ -1,0 +1,64 @@
+"""torch.float64 autograd restatement of render_func_rays -- TEST INFRASTRUCTURE ONLY.
+
+Stands in for "JAX autodiff of the baseline" (jax is not available): the same per-pair
+math as baseline/fm_render.py:11-109 (and oracle/fm_oracle.py) written with torch ops so
+reverse-mode autodiff yields reference gradients with respect to the means, the upper
+triangle of the precision factor and the log-weights.  Only tests and the smoke check use it.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+
+def render_rays_torch(rays, starts, means, prec_full, weights_log, beta_2, beta_3):
+    """(depth, alpha) for rays (P,3) starting at starts (P,3); float64 torch tensors."""
+    prc = torch.triu(prec_full).transpose(1, 2)                     # (N,3,3) lower
+    p = means[None, :, :] - starts[:, None, :]                      # (P,N,3)
+    projp = torch.einsum("nij,pnj->pni", prc, p)
+    pr = torch.einsum("nij,pj->pni", prc, rays)
+    vsv = (pr * pr).sum(-1)
+    psv = (projp * pr).sum(-1)
+    res = psv / vsv
+    v = rays[:, None, :] * res[..., None] - p
+    pv = torch.einsum("nij,pnj->pni", prc, v)
+    d0 = (pv * pv).sum(-1)
+    d2 = -0.5 * d0 + weights_log[None, :]
+    alpha = 1 - torch.exp(-torch.exp(d2).sum(1))
+    sig1 = (res > 0).to(res.dtype)
+    logit = -res * beta_2 + beta_3 * d2
+    se = torch.exp(logit - logit.max(1, keepdim=True).values)
+    w = sig1 * torch.nan_to_num(se) + 1e-20
+    w = w / w.sum(1, keepdim=True)
+    depth = (w * torch.nan_to_num(res)).sum(1)
+    return depth, alpha
+
+
+def pinhole_rays(cam, pixels=None):
+    H, W = cam.H, cam.W
+    idx = np.arange(H * W) if pixels is None else np.asarray(pixels)
+    x = (idx % W).astype(np.float64)
+    y = (idx // W).astype(np.float64)
+    rc = np.stack([(x - cam.cx) / cam.fx, (y - cam.cy) / cam.fy, np.ones_like(x)], 1)
+    return rc @ np.asarray(cam.R, dtype=np.float64)
+
+
+def grads(scene, cams, d_depth, d_alpha, chunk=2048):
+    """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> (float64).
+
+    Returns (d_means (N,3), d_prec (N,3,3) -- upper triangle, d_weights_log (N,))."""
+    means = torch.tensor(scene.means, dtype=torch.float64, requires_grad=True)
+    prec = torch.tensor(scene.prec, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(scene.weights_log, dtype=torch.float64, requires_grad=True)
+    for v, cam in enumerate(cams):
+        rays_all = pinhole_rays(cam)
+        gd = np.asarray(d_depth[v], dtype=np.float64).reshape(-1)
+        ga = np.asarray(d_alpha[v], dtype=np.float64).reshape(-1)
+        for s in range(0, rays_all.shape[0], chunk):
+            e = min(rays_all.shape[0], s + chunk)
+            rays = torch.tensor(rays_all[s:e])
+            starts = torch.tensor(np.repeat(np.asarray(cam.t, dtype=np.float64)[None], e - s, 0))
+            d, a = render_rays_torch(rays, starts, means, prec, w, scene.beta2, scene.beta3)
+            loss = (d * torch.tensor(gd[s:e])).sum() + (a * torch.tensor(ga[s:e])).sum()
+            loss.backward()
+    return means.grad.numpy(), np.triu(prec.grad.numpy()), w.grad.numpy()

@@ -49,7 +49,7 @@ def load():
     lib.fe_ref_tiles.restype = ctypes.c_int64
     lib.fe_ref_tiles.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                                 ctypes.c_float, ctypes.c_void_p, _u32p, _u32p]
+                                 ctypes.c_float, ctypes.c_float, ctypes.c_void_p, _u32p, _u32p]
     _lib = lib
     return lib
 
@@ -108,9 +108,10 @@ def records(scene, cam):
 
 
 def cull_constants(n):
-    """(c_alpha, delta2) exactly as the planner computes them (float32 of float64 math)."""
+    """(c_alpha, delta2, floor_thr) exactly as the planner computes them (float32 of float64)."""
     ln_n = np.log(max(n, 1))
-    return float(np.float32(ln_n + 14.0)), float(np.float32((ln_n + 20.0) * LOG2E))
+    return (float(np.float32(ln_n + 14.0)), float(np.float32((ln_n + 20.0) * LOG2E)),
+            float(np.float32(13.0 * 3.321928094887362 - ln_n * LOG2E)))
 
 
 def tiles(scene, cam, ts, tile_lmin=None):
@@ -123,12 +124,13 @@ def tiles(scene, cam, ts, tile_lmin=None):
     T = tx * ty
     counts = np.zeros(T, np.uint32)
     lst = np.zeros(max(n * T, 1), np.uint32)
-    c_alpha, delta2 = cull_constants(n)
+    c_alpha, delta2, floor_thr = cull_constants(n)
     lm = None
     if tile_lmin is not None:
         lm_arr = np.ascontiguousarray(tile_lmin, np.float32)
         lm = lm_arr.ctypes.data_as(ctypes.c_void_p)
     total = lib.fe_ref_tiles(n, np.ascontiguousarray(rec.reshape(-1) if n else np.zeros(16, np.float32)), deg,
                              cam.H, cam.W, ts, np.float32(cam.cx), np.float32(cam.cy), np.float32(scene.beta2),
-                             np.float32(scene.beta3), np.float32(c_alpha), np.float32(delta2), lm, counts, lst)
+                             np.float32(scene.beta3), np.float32(c_alpha), np.float32(delta2),
+                             np.float32(floor_thr), lm, counts, lst)
     return counts, lst[:total].copy()

@@ -1,0 +1,55 @@
+"""Backward parity on the GPU: hand-written backward kernel (fe_backward) vs float64 autograd
+of the restated reference math (oracle/grad_oracle.py).  Tolerance: max abs error of each
+gradient array <= 1e-3 of that array's max magnitude (float32 accumulation over pixels)."""
+import numpy as np
+import pytest
+
+from oracle import grad_oracle as go
+
+pytestmark = pytest.mark.gpu
+
+GRAD_RTOL = 1e-3
+
+
+def _gpu_grads(fea, scene, cams, gds, gas, mode, tile=16, batch=64):
+    import torch
+    r = fea.Renderer(tile=tile, batch=batch, mode=mode)
+    st = fea.SceneTensors.from_scene(scene)
+    r.forward([st], [(0, c, scene.beta2, scene.beta3) for c in cams], save_for_backward=True)
+    g = r.backward([torch.tensor(x.reshape(c.H, c.W), dtype=torch.float32, device="cuda") for x, c in zip(gds, cams)],
+                   [torch.tensor(x.reshape(c.H, c.W), dtype=torch.float32, device="cuda") for x, c in zip(gas, cams)])
+    torch.cuda.synchronize()
+    dm, dp, dw = [x.cpu().numpy().astype(np.float64) for x in g[0]]
+    return dm, np.triu(dp), dw
+
+
+def _rel(a, b):
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+
+
+@pytest.mark.parametrize("mode,tile", [("dense", 16), ("culled", 16), ("culled", 8), ("culled", 32)])
+def test_backward_single_view(fea, scenes_mod, mode, tile):
+    scene = scenes_mod.single_view_scene()
+    cam = scene.cameras[0]
+    rng = np.random.default_rng(31)
+    gd = rng.normal(size=cam.H * cam.W)
+    ga = rng.normal(size=cam.H * cam.W) * 10
+    ref = go.grads(scene, [cam], [gd], [ga])
+    got = _gpu_grads(fea, scene, [cam], [gd], [ga], mode, tile=tile)
+    errs = [_rel(g, r) for g, r in zip(got, ref)]
+    print(f"backward {mode} tile={tile}: rel err means {errs[0]:.2e} prec {errs[1]:.2e} w {errs[2]:.2e}")
+    assert max(errs) <= GRAD_RTOL
+
+
+def test_backward_two_views_1024(fea, scenes_mod):
+    """Silhouette-workload shapes (N=1024) at reduced resolution, two views summed."""
+    init, _target = scenes_mod.silhouette_problem(n_views=16, H=120, W=160, f=150.0)
+    cams = [init.cameras[2], init.cameras[9]]
+    rng = np.random.default_rng(32)
+    gds = [rng.normal(size=c.H * c.W) for c in cams]
+    gas = [rng.normal(size=c.H * c.W) * 10 for c in cams]
+    ref = go.grads(init, cams, gds, gas)
+    got = _gpu_grads(fea, init, cams, gds, gas, "culled")
+    errs = [_rel(g, r) for g, r in zip(got, ref)]
+    print(f"backward N=1024 2 views: rel err means {errs[0]:.2e} prec {errs[1]:.2e} w {errs[2]:.2e}")
+    assert max(errs) <= GRAD_RTOL

@@ -42,7 +42,7 @@ def test_single_view_full_image(fea, scenes_mod, mode, tile):
     print(f"single view {mode} tile={tile}: max depth rel err {ed:.3e}, max alpha abs err {ea:.3e}")
     assert ea <= ALPHA_TOL and ed <= DEPTH_RTOL
     if mode == "culled":
-        assert r.stats()["floor_tiles"] == 0
+        assert r.stats()["floor_unresolved"] == 0
 
 
 @pytest.mark.parametrize("view", [0, 5, 11])
@@ -65,7 +65,7 @@ def test_shell_view_sampled_pixels(fea, scenes_mod, view):
     ed, ea = _errors(d, a, od, oa)
     print(f"shell view {view}: max depth rel err {ed:.3e}, max alpha abs err {ea:.3e}, stats {stats}")
     assert ea <= ALPHA_TOL and ed <= DEPTH_RTOL
-    assert stats["floor_tiles"] == 0
+    assert stats["floor_unresolved"] == 0 and stats["floor_tiles"] == 0
 
 
 def test_shell_culled_matches_dense_full_view(fea, scenes_mod):

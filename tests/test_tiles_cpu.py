@@ -43,7 +43,7 @@ def test_phase1_and_phase2_are_conservative(scenes_mod, seed, ts):
     scene = sc.single_view_scene(seed=seed, n=60, H=64, W=80, f=64.0)
     cam = scene.cameras[0]
     zs, d2 = _pairs(scene, cam)
-    c_alpha, delta2 = ref.cull_constants(scene.n)
+    c_alpha, delta2, _ = ref.cull_constants(scene.n)
     T = ((cam.W + ts - 1) // ts) * ((cam.H + ts - 1) // ts)
     counts, lst = ref.tiles(scene, cam, ts)
     assert counts.sum() == len(lst)
