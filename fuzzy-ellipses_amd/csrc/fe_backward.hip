@@ -27,44 +27,6 @@ __global__ void k_zero(float* __restrict__ p, int64_t n) {
     p[i] = 0.0f;
 }
 
-// Reduce 16 per-lane values over a wave64 with 17 cross-lane exchanges.  Afterwards lane
-// 4*k holds the wave total of value perm(k), where perm(k) is the value index selected by
-// lane bits 5..2; the function returns that index through *idx.
-__device__ __forceinline__ float wave_reduce16(float (&v)[16], int lane, int* idx) {
-  const bool b5 = (lane >> 5) & 1, b4 = (lane >> 4) & 1, b3 = (lane >> 3) & 1, b2 = (lane >> 2) & 1;
-  float a8[8];
-#pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const float mine = b5 ? v[i + 8] : v[i];
-    const float other = b5 ? v[i] : v[i + 8];
-    a8[i] = mine + __shfl_xor(other, 32, 64);
-  }
-  float a4[4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const float mine = b4 ? a8[i + 4] : a8[i];
-    const float other = b4 ? a8[i] : a8[i + 4];
-    a4[i] = mine + __shfl_xor(other, 16, 64);
-  }
-  float a2[2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const float mine = b3 ? a4[i + 2] : a4[i];
-    const float other = b3 ? a4[i] : a4[i + 2];
-    a2[i] = mine + __shfl_xor(other, 8, 64);
-  }
-  float a1;
-  {
-    const float mine = b2 ? a2[1] : a2[0];
-    const float other = b2 ? a2[0] : a2[1];
-    a1 = mine + __shfl_xor(other, 4, 64);
-  }
-  a1 += __shfl_xor(a1, 2, 64);
-  a1 += __shfl_xor(a1, 1, 64);
-  *idx = (b5 ? 8 : 0) + (b4 ? 4 : 0) + (b3 ? 2 : 0) + (b2 ? 1 : 0);
-  return a1;
-}
-
 struct BwdArgs {
   const Job* jobs;
   const BwdJob* bjobs;
@@ -79,13 +41,17 @@ struct BwdArgs {
   int dense;
 };
 
-template <int TS, int GB>
-__global__ __launch_bounds__(TS* TS) void k_bwd_blend(BwdArgs A) {
-  constexpr int NT = TS * TS;
-  constexpr int NW = NT / 64;
-  __shared__ float4 lds[GB * 4];
-  __shared__ float acc[GB * 16];
-  __shared__ int gid_sh[GB];
+// One block (4 waves) per tile.  Lane = one Gaussian of the tile's lists (phase 1 then
+// phase 2, or all Gaussians in dense mode); waves take chunks of 64 Gaussians round-robin.
+// Each lane loops over the tile's pixels, whose per-pixel terms (x', y', M, D, cE, cD) are
+// staged once in LDS and read as wave-uniform broadcasts, and keeps its Gaussian's 13 sums
+// in registers -- no cross-lane reduction.  The sums go to the per-(Gaussian, view)
+// accumulator with one atomic add each.
+template <int TS>
+__global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
+  constexpr int NPIX = TS * TS;
+  __shared__ float4 pxa[NPIX];   // x', y', M, D
+  __shared__ float2 pxb[NPIX];   // cE, cD
 
   const int64_t gt = blockIdx.x;
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
@@ -93,17 +59,8 @@ __global__ __launch_bounds__(TS* TS) void k_bwd_blend(BwdArgs A) {
   const BwdJob& bj = A.bjobs[ji];
   const int tile = (int)(gt - jb.tile_base);
   const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
-  const int px = tx * TS + (threadIdx.x % TS);
-  const int py = ty * TS + (threadIdx.x / TS);
-  const bool inimg = (px < jb.W) && (py < jb.H);
-  const int64_t lp = (int64_t)(inimg ? py : 0) * jb.W + (inimg ? px : 0);
-  const int64_t pix = jb.pix_base + lp;
-  const float xf = (float)px - jb.cx;
-  const float yf = (float)py - jb.cy;
-  const float b3 = jb.beta3, b2 = jb.beta2;
-  const float nb2l = -jb.beta2 * kLog2e;
-  const float kd = -0.5f * kLog2e;
-  const int lane = threadIdx.x & 63;
+  const int wv = min(TS, jb.W - tx * TS), hv = min(TS, jb.H - ty * TS);   // valid extent
+  const int npx = wv * hv;
 
   int64_t s1, n1, s2, n2;
   if (A.dense) { s1 = 0; n1 = jb.n; s2 = 0; n2 = 0; }
@@ -111,41 +68,58 @@ __global__ __launch_bounds__(TS* TS) void k_bwd_blend(BwdArgs A) {
     s1 = (int64_t)A.off1[gt]; n1 = (int64_t)A.off1[gt + 1] - s1;
     s2 = (int64_t)A.off2[gt]; n2 = (int64_t)A.off2[gt + 1] - s2;
   }
-  const float ncnt = (float)(n1 + n2);
-  const float M = A.st[pix];
-  const float S = A.st[A.total_pix + pix];
-  const float E = A.st[3 * A.total_pix + pix];
-  const float D = A.st[5 * A.total_pix + pix];
-  const float gD = (inimg && bj.gD) ? bj.gD[lp] : 0.0f;
-  const float gA = (inimg && bj.gA) ? bj.gA[lp] : 0.0f;
-  const float cE = gA * expf(-E);
-  const float den = S + 1e-20f * ncnt;
-  const float cD = (den > 0.0f) ? gD / den : 0.0f;
-
   const int64_t total = n1 + n2;
+  if (total == 0) return;
+  const float ncnt = (float)total;
+
+  // per-pixel terms, packed over the valid pixels of the tile
+  for (int i = threadIdx.x; i < npx; i += 256) {
+    const int px = tx * TS + (i % wv), py = ty * TS + (i / wv);
+    const int64_t lp = (int64_t)py * jb.W + px;
+    const int64_t pix = jb.pix_base + lp;
+    const float M = A.st[pix];
+    const float S = A.st[A.total_pix + pix];
+    const float E = A.st[3 * A.total_pix + pix];
+    const float D = A.st[5 * A.total_pix + pix];
+    const float gD = bj.gD ? bj.gD[lp] : 0.0f;
+    const float gA = bj.gA ? bj.gA[lp] : 0.0f;
+    const float den = S + 1e-20f * ncnt;
+    float cE = gA * expf(-E);
+    float cD = (den > 0.0f) ? gD / den : 0.0f;
+    if (!(cE == cE)) cE = 0.0f;
+    if (!(cD == cD) || !(D == D)) cD = 0.0f;
+    pxa[i] = make_float4((float)px - jb.cx, (float)py - jb.cy, M, D);
+    pxb[i] = make_float2(cE, cD);
+  }
+  __syncthreads();
+
+  const float b3 = jb.beta3, b2 = jb.beta2;
+  const float nb2l = -jb.beta2 * kLog2e;
+  const float kd = -0.5f * kLog2e;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
-  for (int i = threadIdx.x; i < GB * 16; i += NT) acc[i] = 0.0f;
-  for (int64_t base = 0; base < total; base += GB) {
-    const int nb = (int)min<int64_t>(GB, total - base);
-    for (int i = threadIdx.x; i < nb * 4; i += NT) {
-      const int k = i >> 2, part = i & 3;
-      const int64_t kk = base + k;
-      int64_t gid;
-      if (A.dense) gid = kk;
-      else gid = (kk < n1) ? (int64_t)A.list[s1 + kk] : (int64_t)A.list[s2 + (kk - n1)];
-      lds[i] = rec4[gid * 4 + part];
-      if (part == 0) gid_sh[k] = (int)gid;
-    }
-    __syncthreads();
-    for (int k = 0; k < nb; ++k) {
-      const float4 r0 = lds[k * 4 + 0];
-      const float4 r1 = lds[k * 4 + 1];
-      const float4 r2 = lds[k * 4 + 2];
-      const float4 r3 = lds[k * 4 + 3];
-      const float u0 = fmaf(r0.x, xf, fmaf(r0.w, yf, r1.z));
-      const float u1 = fmaf(r0.y, xf, fmaf(r1.x, yf, r1.w));
-      const float u2 = fmaf(r0.z, xf, fmaf(r1.y, yf, r2.x));
-      const float q0 = r2.y, q1 = r2.z, q2 = r2.w;
+
+  for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 256) {
+    const int64_t kk = c0 + lane;
+    if (kk >= total) break;
+    int64_t gid;
+    if (A.dense) gid = kk;
+    else gid = (kk < n1) ? (int64_t)A.list[s1 + kk] : (int64_t)A.list[s2 + (kk - n1)];
+    const float4 r0 = rec4[gid * 4 + 0];
+    const float4 r1 = rec4[gid * 4 + 1];
+    const float4 r2 = rec4[gid * 4 + 2];
+    const float4 r3 = rec4[gid * 4 + 3];
+    const float a0 = r0.x, a1 = r0.y, a2 = r0.z, bb0 = r0.w, bb1 = r1.x, bb2 = r1.y;
+    const float g0 = r1.z, g1 = r1.w, g2 = r2.x, q0 = r2.y, q1 = r2.z, q2 = r2.w, wl = r3.y;
+    float Gq0 = 0.f, Gq1 = 0.f, Gq2 = 0.f, Ga0 = 0.f, Ga1 = 0.f, Ga2 = 0.f, Gb0 = 0.f, Gb1 = 0.f, Gb2 = 0.f;
+    float Gg0 = 0.f, Gg1 = 0.f, Gg2 = 0.f, Gw = 0.f;
+    for (int p = 0; p < npx; ++p) {
+      const float4 pa = pxa[p];
+      const float2 pb = pxb[p];
+      const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y;
+      const float u0 = fmaf(a0, xf, fmaf(bb0, yf, g0));
+      const float u1 = fmaf(a1, xf, fmaf(bb1, yf, g1));
+      const float u2 = fmaf(a2, xf, fmaf(bb2, yf, g2));
       const float vsv = fmaf(u0, u0, fmaf(u1, u1, u2 * u2));
       const float psv = fmaf(q0, u0, fmaf(q1, u1, q2 * u2));
       const float iv = __builtin_amdgcn_rcpf(vsv);
@@ -154,7 +128,7 @@ __global__ __launch_bounds__(TS* TS) void k_bwd_blend(BwdArgs A) {
       const float v1 = fmaf(t, u1, -q1);
       const float v2 = fmaf(t, u2, -q2);
       const float d0 = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
-      const float d2l = fmaf(kd, d0, r3.y);
+      const float d2l = fmaf(kd, d0, wl);
       const float e = __builtin_amdgcn_exp2f(d2l);
       const float l = fmaf(b3, d2l, nb2l * t);
       float s = __builtin_amdgcn_exp2f(l - M);
@@ -162,40 +136,28 @@ __global__ __launch_bounds__(TS* TS) void k_bwd_blend(BwdArgs A) {
       const float cs = cD * s;
       const float dt = t - D;
       float gd2 = fmaf(b3 * cs, dt, cE * e);
-      float gt_ = cs * fmaf(-b2, dt, 1.0f);
-      if (!inimg || !(gd2 == gd2) || !(gt_ == gt_)) { gd2 = 0.0f; gt_ = 0.0f; }
-      const float gtv = gt_ * iv;
+      float gtv = cs * fmaf(-b2, dt, 1.0f) * iv;
+      if (!(gd2 == gd2) || !(gtv == gtv)) { gd2 = 0.0f; gtv = 0.0f; }
+      Gq0 = fmaf(gd2, v0, fmaf(gtv, u0, Gq0));
+      Gq1 = fmaf(gd2, v1, fmaf(gtv, u1, Gq1));
+      Gq2 = fmaf(gd2, v2, fmaf(gtv, u2, Gq2));
       const float m2t = -2.0f * t;
       const float gdt = gd2 * t;
-      float val[16];
-      val[0] = fmaf(gd2, v0, gtv * u0);
-      val[1] = fmaf(gd2, v1, gtv * u1);
-      val[2] = fmaf(gd2, v2, gtv * u2);
       const float gu0 = fmaf(gtv, fmaf(m2t, u0, q0), -(gdt * v0));
       const float gu1 = fmaf(gtv, fmaf(m2t, u1, q1), -(gdt * v1));
       const float gu2 = fmaf(gtv, fmaf(m2t, u2, q2), -(gdt * v2));
-      val[3] = gu0 * xf; val[4] = gu1 * xf; val[5] = gu2 * xf;
-      val[6] = gu0 * yf; val[7] = gu1 * yf; val[8] = gu2 * yf;
-      val[9] = gu0; val[10] = gu1; val[11] = gu2;
-      val[12] = gd2;
-      val[13] = 0.0f; val[14] = 0.0f; val[15] = 0.0f;
-      int idx;
-      const float red = wave_reduce16(val, lane, &idx);
-      if ((lane & 3) == 0 && idx < 13) atomicAdd(&acc[k * 16 + idx], red);
+      Ga0 = fmaf(gu0, xf, Ga0); Ga1 = fmaf(gu1, xf, Ga1); Ga2 = fmaf(gu2, xf, Ga2);
+      Gb0 = fmaf(gu0, yf, Gb0); Gb1 = fmaf(gu1, yf, Gb1); Gb2 = fmaf(gu2, yf, Gb2);
+      Gg0 += gu0; Gg1 += gu1; Gg2 += gu2;
+      Gw += gd2;
     }
-    __syncthreads();
-    // flush this batch's per-Gaussian sums
-    for (int i = threadIdx.x; i < nb * 16; i += NT) {
-      const int k = i >> 4, c = i & 15;
-      if (c < 13) {
-        const float vsum = acc[i];
-        if (vsum != 0.0f) atomicAdd(A.gacc + (jb.rec_base + gid_sh[k]) * 16 + c, vsum);
-      }
-      acc[i] = 0.0f;
-    }
-    __syncthreads();
+    float* G = A.gacc + (jb.rec_base + gid) * 16;
+    atomicAdd(G + 0, Gq0); atomicAdd(G + 1, Gq1); atomicAdd(G + 2, Gq2);
+    atomicAdd(G + 3, Ga0); atomicAdd(G + 4, Ga1); atomicAdd(G + 5, Ga2);
+    atomicAdd(G + 6, Gb0); atomicAdd(G + 7, Gb1); atomicAdd(G + 8, Gb2);
+    atomicAdd(G + 9, Gg0); atomicAdd(G + 10, Gg1); atomicAdd(G + 11, Gg2);
+    atomicAdd(G + 12, Gw);
   }
-  (void)NW;
 }
 
 // Per (Gaussian, view): chain rule from the 13 sums to d(mu), d(U upper), d(w).
@@ -247,21 +209,10 @@ hipError_t launch_zero(hipStream_t s, float* p, int64_t n) {
   return hipGetLastError();
 }
 
-template <int TS, int GB>
-static hipError_t bwd_tg(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
-  hipLaunchKernelGGL((k_bwd_blend<TS, GB>), dim3((unsigned)total_tiles), dim3(TS * TS), 0, s, a);
-  return hipGetLastError();
-}
-
 template <int TS>
-static hipError_t bwd_t(hipStream_t s, int gb, int64_t total_tiles, const BwdArgs& a) {
-  switch (gb) {
-    case 32: return bwd_tg<TS, 32>(s, total_tiles, a);
-    case 64: return bwd_tg<TS, 64>(s, total_tiles, a);
-    case 128: return bwd_tg<TS, 128>(s, total_tiles, a);
-    case 256: return bwd_tg<TS, 256>(s, total_tiles, a);
-  }
-  return hipErrorInvalidValue;
+static hipError_t bwd_t(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
+  hipLaunchKernelGGL((k_bwd_blend<TS>), dim3((unsigned)total_tiles), dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int gb, int64_t total_tiles, const Job* jobs,
@@ -269,10 +220,11 @@ hipError_t launch_bwd_blend(hipStream_t s, int ts, int gb, int64_t total_tiles, 
                             const unsigned long long* off2, const uint32_t* list, const float* st,
                             int64_t total_pix, float* gacc, int dense) {
   BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, off1, off2, list, st, total_pix, gacc, dense};
+  (void)gb;
   switch (ts) {
-    case 8: return bwd_t<8>(s, gb, total_tiles, a);
-    case 16: return bwd_t<16>(s, gb, total_tiles, a);
-    case 32: return bwd_t<32>(s, gb, total_tiles, a);
+    case 8: return bwd_t<8>(s, total_tiles, a);
+    case 16: return bwd_t<16>(s, total_tiles, a);
+    case 32: return bwd_t<32>(s, total_tiles, a);
   }
   return hipErrorInvalidValue;
 }

@@ -42,6 +42,7 @@ struct Layout {
   int n_jobs = 0;
   int max_n = 0;
   int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0;
+  int64_t total_stiles = 0;
   int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
@@ -84,7 +85,9 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->save = o.save_for_backward ? 1 : 0;
   L->n_jobs = n_views;
   if (jobs) jobs->resize(n_views);
-  int64_t rec = 0, tiles = 0, maskw = 0, pix = 0, cap = 0;
+  int64_t rec = 0, tiles = 0, stiles = 0, maskw = 0, pix = 0, cap = 0;
+  // membership pre-test groups cover about 32x32 pixels
+  const int sf = (o.tile == 8) ? 4 : (o.tile == 16 ? 2 : 1);
   int max_n = 0;
   for (int v = 0; v < n_views; ++v) {
     const fe_view& vw = views[v];
@@ -125,6 +128,10 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       jb.n = sc.n; jb.H = c.height; jb.W = c.width;
       jb.tiles_x = tx; jb.tiles_y = ty; jb.n_tiles = tx * ty;
       jb.nw = nw;
+      jb.sf = sf;
+      jb.stiles_x = (tx + sf - 1) / sf;
+      jb.stiles_y = (ty + sf - 1) / sf;
+      jb.stile_base = stiles;
       jb.scene = vw.scene;
       jb.rec_base = rec;
       jb.tile_base = tiles;
@@ -134,6 +141,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     max_n = std::max(max_n, sc.n);
     rec += sc.n;
     tiles += (int64_t)tx * ty;
+    stiles += (int64_t)((tx + sf - 1) / sf) * ((ty + sf - 1) / sf);
     maskw += (int64_t)tx * ty * nw;
     pix += (int64_t)c.height * c.width;
     cap += (int64_t)tx * ty * sc.n;
@@ -145,6 +153,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->max_n = max_n;
   L->total_rec = rec;
   L->total_tiles = tiles;
+  L->total_stiles = stiles;
   L->total_mask = maskw;
   L->total_pix = pix;
   L->list_cap = cap;
@@ -308,7 +317,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   uint32_t* list = at<uint32_t>(ws, L.o_list);
   // phase 1
   FE_HIP(timed(s, FE_K_MASK1, [&] {
-    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
+    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
                             cnt);
   }));
   FE_HIP(timed(s, FE_K_SCAN1, [&] { return launch_scan(s, count, L.total_tiles, partial, off1, nullptr); }));
@@ -319,7 +328,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   }));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
-    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_tiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
+    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
                             cnt);
   }));
   FE_HIP(timed(s, FE_K_SCAN2, [&] {

@@ -41,6 +41,9 @@ struct Job {
   int32_t n, H, W;
   int32_t tiles_x, tiles_y, n_tiles;
   int32_t nw;      // 64-bit mask words per tile = ceil(n / 64)
+  int32_t sf;      // super-tile factor: membership is pre-tested on sf x sf groups of tiles
+  int32_t stiles_x, stiles_y;
+  int64_t stile_base; // first global super-tile
   int32_t scene;
   int64_t rec_base;   // first record (Gaussian-view) of this job
   int64_t tile_base;  // first global tile
@@ -95,8 +98,7 @@ struct TileGeo {
   float Xc, Yc, hx, hy;
 };
 
-__device__ __forceinline__ TileGeo tile_geo(const Job& jb, int tile, int ts) {
-  int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
+__device__ __forceinline__ TileGeo tile_geo_xy(const Job& jb, int tx, int ty, int ts) {
   int x0 = tx * ts, y0 = ty * ts;
   int xl = min(x0 + ts, jb.W) - 1, yl = min(y0 + ts, jb.H) - 1;
   TileGeo g;

@@ -103,15 +103,22 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
 }
 
 // ------------------------------------------------------------------------------------------
-// (b) tile membership: one block (4 waves) per tile, each wave tests 64 Gaussians at a time.
+// (b) tile membership.  One block (4 waves) per super-tile of sf x sf tiles; each wave tests
+// 64 Gaussians at a time.  A lane loads its Gaussian's record once, bounds it over the whole
+// super-tile and, only if that bound keeps it, over each tile of the group (a pair the
+// group bound rejects is rejected for every tile in it: the group bound is a bound over a
+// superset of rays).  Each tile's decisions are ballot-packed into 64-bit membership words
+// and counted into the tile histogram.
 // PHASE 1 keeps pairs whose density bound reaches exp(-c_alpha):  d0_lo/2 - w <= c_alpha,
 //   writing mask1.
 // PHASE 2 keeps pairs absent from mask1 whose depth-logit bound reaches the tile threshold
 //   tau = min_pixels(log2-sum-exp of phase-1 logits) - delta2, writing mask2.  It also takes
 //   the largest logit bound `ubm` of the dropped pairs that might lie behind the camera: if
 //   such a pair could raise the max logit so far that the 1e-20 weight floor becomes visible
-//   (ubm - lmin >= log2(1e13 / n)), the tile is completed densely (every remaining pair).
+//   (ubm - lmin >= floor_thr), the tile is completed densely (every remaining pair).
 // ------------------------------------------------------------------------------------------
+constexpr int kMaxSF = 4;
+
 template <int PHASE>
 __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs, int n_jobs, int ts,
                                                    const float* __restrict__ rec,
@@ -122,85 +129,153 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
                                                    float* __restrict__ tile_ubm,
                                                    const int32_t* __restrict__ job_flags,
                                                    Counters* __restrict__ cnt_out) {
-  __shared__ uint32_t sh_cnt[4];
-  __shared__ float sh_ubm[4];
-  const int64_t gt = blockIdx.x;
-  const int ji = find_job(n_jobs, gt, [&](int k) { return jobs[k].tile_base; });
+  constexpr int NS = kMaxSF * kMaxSF;
+  __shared__ uint32_t sh_cnt[4][NS];
+  __shared__ float sh_ubm[4][NS];
+  const int64_t gs = blockIdx.x;
+  const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
   const Job& jb = jobs[ji];
-  const int tile = (int)(gt - jb.tile_base);
-  const TileGeo tg = tile_geo(jb, tile, ts);
+  const int sf = jb.sf;
+  const int st = (int)(gs - jb.stile_base);
+  const int sx = st % jb.stiles_x, sy = st / jb.stiles_x;
+  const TileGeo sg = tile_geo_xy(jb, sx, sy, ts * sf);
+  // sub-tiles of this group that exist
+  const int tx0 = sx * sf, ty0 = sy * sf;
+  const int nsx = min(sf, jb.tiles_x - tx0), nsy = min(sf, jb.tiles_y - ty0);
+  const int nsub = nsx * nsy;
+  // per-sub-tile geometry and thresholds, block-uniform, kept in LDS
+  __shared__ TileGeo tg[NS];
+  __shared__ float tau[NS], lmin[NS];
+  __shared__ float tau_grp;
+  if (threadIdx.x < nsub) {
+    const int k = threadIdx.x;
+    const int txk = tx0 + k % nsx, tyk = ty0 + k / nsx;
+    tg[k] = tile_geo_xy(jb, txk, tyk, ts);
+    if (PHASE == 2) {
+      lmin[k] = tile_lmin[jb.tile_base + (int64_t)tyk * jb.tiles_x + txk];
+      tau[k] = lmin[k] - jb.delta2;
+    }
+  }
+  __syncthreads();
+  if (PHASE == 2 && threadIdx.x == 0) {
+    float m = INFINITY;
+    for (int k = 0; k < nsub; ++k) m = fminf(m, tau[k]);
+    tau_grp = m;
+  }
+  __syncthreads();
+  const float tau_s = (PHASE == 2) ? tau_grp : 0.0f;
   const bool dense_job = (job_flags[ji] & 1) != 0;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
-  const float lmin = (PHASE == 2) ? tile_lmin[gt] : 0.0f;
-  const float tau = lmin - jb.delta2;
-  uint32_t cnt = 0;
-  float ubm = -INFINITY;
-  const int64_t row = jb.mask_base + (int64_t)tile * jb.nw;
+  uint32_t cnt[NS];
+  float ubm[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) { cnt[k] = 0; ubm[k] = -INFINITY; }
+
   for (int c = wave; c < jb.nw; c += 4) {
     const int j = c * 64 + lane;
-    const unsigned long long old = (PHASE == 2) ? mask1[row + c] : 0ull;
-    bool keep = false;
-    const bool listed = (PHASE == 2) && ((old >> lane) & 1ull);
-    if (j < jb.n && !listed) {
+    float r[16];
+    bool valid = j < jb.n;
+    if (valid) {
       const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
-      float r[16];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const float4 t4 = r4[i];
         r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
       }
-      const bool degenerate = (__float_as_int(r[15]) & 1) != 0;
-      if (dense_job || degenerate) {
-        keep = (PHASE == 1);
+    }
+    const bool degenerate = valid && ((__float_as_int(r[15]) & 1) != 0);
+    const bool forced = valid && (dense_job || degenerate);   // kept in phase 1, never in phase 2
+    // group test
+    bool keep_s = false, plus_s = true;
+    float ubm_s = -INFINITY;
+    if (valid && !forced) {
+      const PairBound pb = pair_bound(r, sg);
+      const float w = r[12];
+      if (PHASE == 1) {
+        keep_s = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
       } else {
-        const PairBound pb = pair_bound(r, tg);
-        const float w = r[12];
+        const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
+        const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
+        keep_s = pb.unbounded || (ub >= tau_s);
+        plus_s = pb.plus;
+        if (!pb.plus) ubm_s = fmaf(b3l, dlo, -(b2l * pb.t_any_lb));
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      if (k < nsub) {
+        const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
+        const int64_t row = jb.mask_base + tile * jb.nw;
+        const unsigned long long old = (PHASE == 2) ? mask1[row + c] : 0ull;
+        const bool listed = (PHASE == 2) && ((old >> lane) & 1ull);
+        bool keep = false;
         if (PHASE == 1) {
-          keep = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
-        } else {
-          const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
-          const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
-          keep = pb.unbounded || (ub >= tau);
-          if (!keep && !pb.plus) ubm = fmaxf(ubm, fmaf(b3l, dlo, -(b2l * pb.t_any_lb)));
+          keep = forced;
         }
-      }
-    }
-    const unsigned long long word = __ballot(keep);
-    if (lane == 0) {
-      if (PHASE == 1) mask1[row + c] = word;
-      else mask2[row + c] = word;
-    }
-    cnt += (uint32_t)__popcll(word);
-  }
-  if (PHASE == 2) {
-    for (int o = 32; o > 0; o >>= 1) ubm = fmaxf(ubm, __shfl_xor(ubm, o, 64));
-  }
-  if (lane == 0) { sh_cnt[wave] = cnt; sh_ubm[wave] = ubm; }
-  __syncthreads();
-  uint32_t total = sh_cnt[0] + sh_cnt[1] + sh_cnt[2] + sh_cnt[3];
-  if (PHASE == 2) {
-    const float ubm_t = fmaxf(fmaxf(sh_ubm[0], sh_ubm[1]), fmaxf(sh_ubm[2], sh_ubm[3]));
-    const bool dense_tile = (ubm_t - lmin >= jb.floor_thr);   // false when no pair is in doubt (-inf)
-    if (dense_tile) {
-      // complete the tile: every pair not in phase 1
-      __syncthreads();
-      uint32_t c2 = 0;
-      for (int c = wave; c < jb.nw; c += 4) {
-        const int j = c * 64 + lane;
-        const bool keep = (j < jb.n) && !((mask1[row + c] >> lane) & 1ull);
+        if (valid && !forced && !listed) {
+          if (keep_s) {
+            const PairBound pb = pair_bound(r, tg[k]);
+            const float w = r[12];
+            if (PHASE == 1) {
+              keep = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
+            } else {
+              const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
+              const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
+              keep = pb.unbounded || (ub >= tau[k]);
+              if (!keep && !pb.plus) ubm[k] = fmaxf(ubm[k], fmaf(b3l, dlo, -(b2l * pb.t_any_lb)));
+            }
+          } else if (PHASE == 2 && !plus_s) {
+            ubm[k] = fmaxf(ubm[k], ubm_s);
+          }
+        }
         const unsigned long long word = __ballot(keep);
-        if (lane == 0) mask2[row + c] = word;
-        c2 += (uint32_t)__popcll(word);
+        if (lane == 0) {
+          if (PHASE == 1) mask1[row + c] = word;
+          else mask2[row + c] = word;
+        }
+        cnt[k] += (uint32_t)__popcll(word);
       }
-      if (lane == 0) sh_cnt[wave] = c2;
-      __syncthreads();
-      total = sh_cnt[0] + sh_cnt[1] + sh_cnt[2] + sh_cnt[3];
-      if (threadIdx.x == 0) atomicAdd(&cnt_out->floor_tiles, 1ull);
     }
-    if (threadIdx.x == 0) tile_ubm[gt] = dense_tile ? -INFINITY : ubm_t;
   }
-  if (threadIdx.x == 0) count[gt] = total;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    if (PHASE == 2) {
+      float u = ubm[k];
+      for (int o = 32; o > 0; o >>= 1) u = fmaxf(u, __shfl_xor(u, o, 64));
+      ubm[k] = u;
+    }
+    if (lane == 0) { sh_cnt[wave][k] = cnt[k]; sh_ubm[wave][k] = ubm[k]; }
+  }
+  __syncthreads();
+  // one wave finishes the sub-tiles: lane k owns sub-tile k
+  if (wave == 0) {
+    for (int k = 0; k < nsub; ++k) {
+      const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
+      const int64_t gt = jb.tile_base + tile;
+      uint32_t total = sh_cnt[0][k] + sh_cnt[1][k] + sh_cnt[2][k] + sh_cnt[3][k];
+      if (PHASE == 2) {
+        const float ubm_t = fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k]));
+        const bool dense_tile = (ubm_t - lmin[k] >= jb.floor_thr);   // false when nothing in doubt
+        if (dense_tile) {
+          // complete the tile: every pair not in phase 1
+          const int64_t row = jb.mask_base + tile * jb.nw;
+          uint32_t c2 = 0;
+          for (int c = 0; c < jb.nw; ++c) {
+            const int j = c * 64 + lane;
+            const bool keep = (j < jb.n) && !((mask1[row + c] >> lane) & 1ull);
+            const unsigned long long word = __ballot(keep);
+            if (lane == 0) mask2[row + c] = word;
+            c2 += (uint32_t)__popcll(word);
+          }
+          total = c2;
+          if (lane == 0) atomicAdd(&cnt_out->floor_tiles, 1ull);
+        }
+        if (lane == 0) tile_ubm[gt] = dense_tile ? -INFINITY : ubm_t;
+      }
+      if (lane == 0) count[gt] = total;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------
@@ -522,15 +597,15 @@ hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max
   return hipGetLastError();
 }
 
-hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_tiles,
+hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
                             int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
                             uint32_t* count, const float* tile_lmin, float* tile_ubm, const int32_t* job_flags,
                             Counters* cnt) {
   if (phase == 1)
-    hipLaunchKernelGGL(k_tile_mask<1>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
+    hipLaunchKernelGGL(k_tile_mask<1>, dim3((unsigned)total_stiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
                        mask2, count, tile_lmin, tile_ubm, job_flags, cnt);
   else
-    hipLaunchKernelGGL(k_tile_mask<2>, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
+    hipLaunchKernelGGL(k_tile_mask<2>, dim3((unsigned)total_stiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
                        mask2, count, tile_lmin, tile_ubm, job_flags, cnt);
   return hipGetLastError();
 }

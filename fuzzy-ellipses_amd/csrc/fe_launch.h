@@ -12,7 +12,7 @@ struct Counters;
 hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs);
 hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec,
                              int32_t* job_flags);
-hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_tiles,
+hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
                             int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
                             uint32_t* count, const float* tile_lmin, float* tile_ubm, const int32_t* job_flags,
                             Counters* cnt);

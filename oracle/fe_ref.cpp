@@ -271,44 +271,85 @@ static Bound bound_of(const float* r, float Xc, float Yc, float hx, float hy) {
 }
 
 // Tile lists of one view.  phase 1: alpha bound.  phase 2 (tile_lmin != NULL): pairs not in
-// phase 1 whose depth-logit bound reaches tile_lmin[t] - delta2.  Writes counts[T] and the
-// concatenated, per-tile ascending Gaussian ids to `list` (capacity n*T); returns the total.
+// phase 1 whose depth-logit bound reaches tile_lmin[t] - delta2.  Membership is pre-tested
+// on groups of sf x sf tiles (sf = 4, 2, 1 for ts = 8, 16, 32): a pair the group bound
+// rejects is rejected for every tile of the group.  Writes counts[T] and the concatenated,
+// per-tile ascending Gaussian ids to `list` (capacity n*T); returns the total.
+struct Geo {
+  float Xc, Yc, hx, hy;
+};
+
+static Geo geo_of(int tx, int ty, int ts, int W, int H, float cx, float cy) {
+  const int x0 = tx * ts, y0 = ty * ts;
+  const int xl = std::min(x0 + ts, W) - 1, yl = std::min(y0 + ts, H) - 1;
+  Geo g;
+  g.Xc = (float)(x0 + xl) * 0.5f - cx;
+  g.Yc = (float)(y0 + yl) * 0.5f - cy;
+  g.hx = (float)(xl - x0) * 0.5f;
+  g.hy = (float)(yl - y0) * 0.5f;
+  return g;
+}
+
 int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, int ts, float cx, float cy,
                      float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
                      const float* tile_lmin, uint32_t* counts, uint32_t* list) {
   const int tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
+  const int sf = (ts == 8) ? 4 : (ts == 16 ? 2 : 1);
   const float b3l = beta3 * kLog2e, b2l = beta2 * kLog2e;
   int64_t pos = 0;
+  std::vector<char> in1(n > 0 ? n : 1, 0);
   for (int tile = 0; tile < tx * ty; ++tile) {
     const int txi = tile % tx, tyi = tile / tx;
-    const int x0 = txi * ts, y0 = tyi * ts;
-    const int xl = std::min(x0 + ts, W) - 1, yl = std::min(y0 + ts, H) - 1;
-    const float Xc = (float)(x0 + xl) * 0.5f - cx;
-    const float Yc = (float)(y0 + yl) * 0.5f - cy;
-    const float hx = (float)(xl - x0) * 0.5f;
-    const float hy = (float)(yl - y0) * 0.5f;
+    const Geo g = geo_of(txi, tyi, ts, W, H, cx, cy);
+    // group of this tile
+    const int sx = txi / sf, sy = tyi / sf;
+    const Geo gg = geo_of(sx, sy, ts * sf, W, H, cx, cy);
+    float tau_grp = INFINITY;
+    if (tile_lmin) {
+      const int nsx = std::min(sf, tx - sx * sf), nsy = std::min(sf, ty - sy * sf);
+      for (int k = 0; k < nsx * nsy; ++k) {
+        const int tk = (sy * sf + k / nsx) * tx + (sx * sf + k % nsx);
+        tau_grp = fminf(tau_grp, tile_lmin[tk] - delta2);
+      }
+    }
     uint32_t c = 0;
     const int64_t start = pos;
     float ubm = -INFINITY;
-    std::vector<char> in1(n > 0 ? n : 1, 0);
     for (int j = 0; j < n; ++j) {
       const float* r = rec + 16 * (size_t)j;
       int32_t fl;
       memcpy(&fl, &r[15], 4);
-      const bool deg = (fl & 1) != 0;
-      const Bound o = bound_of(r, Xc, Yc, hx, hy);
-      const bool k1 = any_degenerate || deg || o.unbounded || (fmaf(0.5f * o.sd_lo, o.sd_lo, -r[12]) <= c_alpha);
-      in1[j] = k1;
-      bool keep;
-      if (!tile_lmin) {
-        keep = k1;
-      } else {
-        const float tau = tile_lmin[tile] - delta2;
-        const float dlo = fmaf(-0.5f * o.sd_lo, o.sd_lo, r[12]);
-        const float ub = fmaf(b3l, dlo, -(b2l * o.t_lb));
-        keep = !k1 && (o.unbounded || ub >= tau);
-        if (!k1 && !keep && !o.plus) ubm = fmaxf(ubm, fmaf(b3l, dlo, -(b2l * o.t_any_lb)));
+      const bool forced = any_degenerate || (fl & 1) != 0;
+      const float w = r[12];
+      bool k1 = forced, keep2 = false;
+      if (!forced) {
+        const Bound og = bound_of(r, gg.Xc, gg.Yc, gg.hx, gg.hy);
+        const bool g1 = og.unbounded || (fmaf(0.5f * og.sd_lo, og.sd_lo, -w) <= c_alpha);
+        bool sub_done = false;
+        Bound o;
+        if (g1) {
+          o = bound_of(r, g.Xc, g.Yc, g.hx, g.hy);
+          sub_done = true;
+          k1 = o.unbounded || (fmaf(0.5f * o.sd_lo, o.sd_lo, -w) <= c_alpha);
+        }
+        if (tile_lmin && !k1) {
+          const float dlo_g = fmaf(-0.5f * og.sd_lo, og.sd_lo, w);
+          const float ub_g = fmaf(b3l, dlo_g, -(b2l * og.t_lb));
+          const bool g2 = og.unbounded || (ub_g >= tau_grp);
+          if (g2) {
+            if (!sub_done) o = bound_of(r, g.Xc, g.Yc, g.hx, g.hy);
+            const float tau = tile_lmin[tile] - delta2;
+            const float dlo = fmaf(-0.5f * o.sd_lo, o.sd_lo, w);
+            const float ub = fmaf(b3l, dlo, -(b2l * o.t_lb));
+            keep2 = o.unbounded || (ub >= tau);
+            if (!keep2 && !o.plus) ubm = fmaxf(ubm, fmaf(b3l, dlo, -(b2l * o.t_any_lb)));
+          } else if (!og.plus) {
+            ubm = fmaxf(ubm, fmaf(b3l, dlo_g, -(b2l * og.t_any_lb)));
+          }
+        }
       }
+      in1[j] = k1;
+      const bool keep = tile_lmin ? keep2 : k1;
       if (keep) {
         list[pos++] = (uint32_t)j;
         ++c;
