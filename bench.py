@@ -84,6 +84,121 @@ def cpu_forward_f32(scene, cam, rows=32):
             "sample": f"{rows} rows of one view, dense float32 forward, {el:.2f} s"}
 
 
+def served(fea, sc, dev, rate=800.0, n=1000, burst=False, max_batch=64, slots=2, seed=4,
+           resolutions=((128, 128), (240, 320), (480, 640)), backward_frac=0.3, scenes=None):
+    """Serving run of the request mix over 64 resident scenes -> stats dict."""
+    from fuzzy_ellipses_amd.serve import Server, poisson_requests
+    scenes = scenes if scenes is not None else sc.resident_scenes()
+    srv = Server(scenes, device=dev, max_batch=max_batch, slots=slots)
+    warm = poisson_requests(scenes, rate, min(64, n), seed=seed + 100, backward_frac=backward_frac,
+                            resolutions=resolutions, device=dev, burst=True)
+    srv.run(warm)
+    reqs = poisson_requests(scenes, rate, n, seed=seed, backward_frac=backward_frac, resolutions=resolutions,
+                            device=dev, burst=burst)
+    st = srv.run(reqs)
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
+
+
+def workload_batched(fea, sc, dev, steps):
+    """256 scenes (N ~ U{16..2048}) with 1-8 views each at mixed resolutions, one call per
+    kernel for the whole batch, vs one call per view."""
+    import torch
+    from fuzzy_ellipses_amd.autotune import Autotuner
+    scenes = sc.batched_scenes()
+    sts = [fea.SceneTensors.from_scene(s, dev) for s in scenes]
+    views = [(i, c, s.beta2, s.beta3) for i, s in enumerate(scenes) for c in s.cameras]
+    at = Autotuner(dev, reps=2)
+    tile, batch = at.tune(sts, views, backward=False)
+    r = fea.Renderer(tile=tile, batch=batch, device=dev)
+    outs = r.alloc_outputs(views)
+    r.forward(sts, views, outs=outs)
+    torch.cuda.synchronize()
+    stats = r.stats()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r.forward(sts, views, outs=outs, reuse_descriptors=True)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    # the same views through a captured HIP graph
+    graph = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream(dev)
+    side.wait_stream(torch.cuda.current_stream(dev))
+    with torch.cuda.stream(side):
+        r.forward(sts, views, outs=outs)
+    torch.cuda.current_stream(dev).wait_stream(side)
+    with torch.cuda.graph(graph):
+        r.forward(sts, views, outs=outs, reuse_descriptors=True)
+    graph.replay()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        graph.replay()
+    torch.cuda.synchronize()
+    el_g = (time.perf_counter() - t0) / steps
+    # plain path: one view per call
+    r1 = fea.Renderer(tile=16, batch=64, device=dev)
+    for (si, cam, b2, b3) in views[:8]:
+        r1.forward([sts[si]], [(0, cam, b2, b3)])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for (si, cam, b2, b3) in views:
+        r1.forward([sts[si]], [(0, cam, b2, b3)])
+    torch.cuda.synchronize()
+    el_1 = time.perf_counter() - t0
+    dense = sum(sts[si].n * cam.H * cam.W for (si, cam, _, _) in views)
+    return {"metric": "batched forward frames/s (256 scenes, mixed N and resolution)",
+            "value": round(len(views) / el, 2), "unit": "frames/s", "n_gpus": 1, "steps": steps,
+            "ms_per_step": round(1e3 * el, 3), "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": "256 scenes N~U{16..2048}, 1-8 views each, resolutions 64^2..480x640",
+                       "views": len(views), "tile": tile, "lds_batch": batch},
+            "graph_replay_frames_per_s": round(len(views) / el_g, 2),
+            "plain_one_view_per_call_frames_per_s": round(len(views) / el_1, 2),
+            "speedup_vs_plain": round(el_1 / el, 2),
+            "evals_per_s": stats["evals"] / el, "dense_equiv_evals_per_s": dense / el,
+            "autotune_sweep_ms": at.table.get(next(iter(at.table)), {}).get("sweep") if at.table else None}
+
+
+def workload_serving(fea, sc, dev, rates, n_per_rate):
+    """Poisson request mix (70% forward, 30% forward+backward) over 64 resident scenes, plus
+    the saturated fwd+bwd throughput at 480x640, N=4096 of the batched server vs the plain
+    one-request-per-launch path."""
+    out = {"metric": "served p50/p99 ms and throughput", "unit": "ms", "n_gpus": 1, "data": "synthetic",
+           "config": {"workload": "64 resident scenes N~U{64..4096}, resolutions 128^2/240x320/480x640, "
+                                  "70% fwd / 30% fwd+bwd, Poisson arrivals"}, "rates": {}}
+    for rate, n in zip(rates, n_per_rate):
+        out["rates"][str(rate)] = served(fea, sc, dev, rate=rate, n=n)
+        plain = served(fea, sc, dev, rate=rate, n=max(n // 4, 50), max_batch=1, slots=1)
+        out["rates"][str(rate)]["plain_p50_ms"] = plain["p50_ms"]
+        out["rates"][str(rate)]["plain_p99_ms"] = plain["p99_ms"]
+        out["rates"][str(rate)]["plain_throughput_rps"] = plain["throughput_rps"]
+    shell = sc.shell_scene(n_views=1)
+    big = [sc.Scene(shell.means, shell.prec, shell.weights_log, [])]
+    sat_b = served(fea, sc, dev, rate=1.0, n=160, burst=True, max_batch=16, slots=2, resolutions=((480, 640),),
+                   backward_frac=1.0, scenes=big)
+    sat_p = served(fea, sc, dev, rate=1.0, n=40, burst=True, max_batch=1, slots=1, resolutions=((480, 640),),
+                   backward_frac=1.0, scenes=big)
+    out["saturated_fwd_bwd_480x640_n4096"] = {"batched_rps": sat_b["throughput_rps"],
+                                              "plain_rps": sat_p["throughput_rps"],
+                                              "ratio": round(sat_b["throughput_rps"] / sat_p["throughput_rps"], 3)}
+    out["value"] = out["rates"][str(rates[-1])]["p99_ms"]
+    out["higher_is_better"] = False
+    return out
+
+
+def workload_silhouette(fea, sc, dev, steps):
+    from fuzzy_ellipses_amd.silhouette import SilhouetteFit
+    init, target = sc.silhouette_problem()
+    fit = SilhouetteFit(init, target, device=dev, lr=1e-2)
+    fit.run(steps=2)
+    res = fit.run(steps=steps)
+    return {"metric": "shape-from-silhouette fwd+bwd+Adam frames/s (N=1024, 16 views 240x320)",
+            "value": round(res["frames_per_s"], 2), "unit": "frames/s", "n_gpus": 1, "steps": steps,
+            "ms_per_step": round(res["ms_per_step"], 3), "higher_is_better": True, "dtype": "f32",
+            "data": "synthetic", "loss_first": res["loss_first"], "loss_last": res["loss_last"],
+            "config": {"workload": "N=1024 cube scene fit to a second scene's silhouettes+depths, 16 views "
+                                   "240x320, BCE + 0.1 L1, Adam lr 1e-2"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,7 +208,22 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mode", default="culled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="shell", choices=["shell", "batched", "serving", "silhouette"])
+    ap.add_argument("--rates", default="50,200,800")
+    ap.add_argument("--requests", default="500,1000,2000")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.add(src, 0.0, out=dst)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               f"(views and scenes shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
@@ -107,6 +237,17 @@ def main():
     import fuzzy_ellipses_amd.scenes as sc
 
     dev = torch.device("cuda:0")
+    if args.workload == "batched":
+        print(json.dumps(workload_batched(fea, sc, dev, args.steps)), flush=True)
+        return
+    if args.workload == "serving":
+        rates = [float(x) for x in args.rates.split(",")]
+        nreq = [int(x) for x in args.requests.split(",")]
+        print(json.dumps(workload_serving(fea, sc, dev, rates, nreq)), flush=True)
+        return
+    if args.workload == "silhouette":
+        print(json.dumps(workload_silhouette(fea, sc, dev, max(args.steps, 200))), flush=True)
+        return
     scene = sc.shell_scene()
     cams = scene.cameras
     views = [(0, c, scene.beta2, scene.beta3) for c in cams]
@@ -187,6 +328,8 @@ def main():
         "kernel_avg_ms": {k: round(v, 5) for k, v in kavg.items() if kt[k][1]},
         "roofline": roof,
     }
+    if not args.no_serving:
+        out["served"] = served(fea, sc, dev, rate=800.0, n=1000)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cams[0])
         out["cpu_forward_f32"] = cpu_forward_f32(scene, cams[0])

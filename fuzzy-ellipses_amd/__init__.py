@@ -82,28 +82,48 @@ def fe_camera(cam: Camera) -> FeCamera:
 
 
 class Renderer:
-    """Batched renderer.  ``tile`` in {8,16,32}, ``batch`` (LDS depth) in {32,64,128,256}."""
+    """Batched renderer over many scenes and views: one launch per kernel per call.
+
+    ``tile`` in {8, 16, 32} pixels, ``batch`` (LDS depth, Gaussians per staged batch) in
+    {32, 64, 128, 256}, ``mode`` "culled" (error-bounded tile culling) or "dense".  The
+    device workspace is owned by the renderer and reused across calls; forward with
+    ``save_for_backward=True`` keeps what ``backward`` needs in it.
+    """
 
     def __init__(self, tile: int = 16, batch: int = 64, mode: str = "culled", device="cuda"):
         self.lib = _lib.load()
         self.device = torch.device(device)
-        self.opt = FeOptions(int(tile), int(batch), MODES[mode], 0)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.tile, self.batch, self.mode = int(tile), int(batch), MODES[mode]
         self._ws: Optional[torch.Tensor] = None
-        self._last = None  # (scenes, views structs, outputs) of the last forward for backward
+        self._last = None       # forward call state kept for backward
+        self._cache = None      # (key, FeScene[], FeView[]) of the last call
+
+    def _opt(self, save: bool, reuse: bool = False) -> FeOptions:
+        return FeOptions(self.tile, self.batch, self.mode, int(save),
+                         _lib.FE_FLAG_REUSE_DESCRIPTORS if reuse else 0)
 
     # -- planning -----------------------------------------------------------------------
     def _structs(self, scenes: Sequence[SceneTensors], views, outs):
+        key = (tuple(id(s) for s in scenes), tuple((id(v[1]), v[0], v[2], v[3]) for v in views),
+               tuple((_ptr(d), _ptr(a)) for d, a in outs))
+        if self._cache is not None and self._cache[0] == key:
+            return self._cache[1], self._cache[2]
         sc = (FeScene * len(scenes))(*[s.fe() for s in scenes])
         vw = (FeView * len(views))()
         for i, (si, cam, b2, b3) in enumerate(views):
             d, a = outs[i]
             vw[i] = FeView(int(si), fe_camera(cam), float(b2), float(b3), _ptr(d), _ptr(a))
+        self._cache = (key, sc, vw)
         return sc, vw
 
     def workspace_bytes(self, scenes, views, save_for_backward=False) -> int:
-        outs = [(None, None)] * len(views)
-        sc, vw = self._structs(scenes, views, outs)
-        opt = FeOptions(self.opt.tile, self.opt.batch, self.opt.mode, int(save_for_backward))
+        sc = (FeScene * len(scenes))(*[s.fe() for s in scenes])
+        vw = (FeView * len(views))()
+        for i, (si, cam, b2, b3) in enumerate(views):
+            vw[i] = FeView(int(si), fe_camera(cam), float(b2), float(b3), None, None)
+        opt = self._opt(save_for_backward)
         nbytes = ctypes.c_size_t(0)
         _lib.check(self.lib.fe_plan(sc, len(scenes), vw, len(views), ctypes.byref(opt), ctypes.byref(nbytes)),
                    "fe_plan")
@@ -111,29 +131,40 @@ class Renderer:
 
     def _workspace(self, nbytes: int) -> torch.Tensor:
         if self._ws is None or self._ws.numel() < nbytes:
+            self._ws = None
             self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
+            self._cache = None
         return self._ws
+
+    def alloc_outputs(self, views):
+        return [(torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device),
+                 torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device))
+                for (_, cam, _, _) in views]
 
     # -- forward ------------------------------------------------------------------------
     def forward(self, scenes: Sequence[SceneTensors], views, save_for_backward: bool = False,
-                outs=None) -> List[Tuple[torch.Tensor, torch.Tensor]]:
+                outs=None, reuse_descriptors: bool = False) -> List[Tuple[torch.Tensor, torch.Tensor]]:
         """Render every view.  ``views`` = [(scene_index, Camera, beta2, beta3), ...].
 
-        Returns [(depth (H,W), alpha (H,W)), ...] float32 device tensors.
+        Returns [(depth (H,W), alpha (H,W)), ...] float32 device tensors.  With
+        ``reuse_descriptors`` (same scenes, views, outputs and options as the previous call)
+        the call only enqueues kernels and may be captured into a HIP graph.
         """
         if outs is None:
-            outs = [(torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device),
-                     torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device))
-                    for (_, cam, _, _) in views]
+            outs = self.alloc_outputs(views)
         sc, vw = self._structs(scenes, views, outs)
-        opt = FeOptions(self.opt.tile, self.opt.batch, self.opt.mode, int(save_for_backward))
+        opt = self._opt(save_for_backward)
         nbytes = ctypes.c_size_t(0)
         _lib.check(self.lib.fe_plan(sc, len(scenes), vw, len(views), ctypes.byref(opt), ctypes.byref(nbytes)),
                    "fe_plan")
+        if reuse_descriptors and (self._ws is None or self._ws.numel() < nbytes.value):
+            raise FeError("forward: reuse_descriptors needs a prior call with the same arguments")
         ws = self._workspace(int(nbytes.value))
+        sc, vw = self._structs(scenes, views, outs)
+        opt = self._opt(save_for_backward, reuse_descriptors)
         _lib.check(self.lib.fe_forward(_stream(self.device), sc, len(scenes), vw, len(views), ctypes.byref(opt),
                                        ws.data_ptr(), ws.numel()), "fe_forward")
-        self._last = (list(scenes), sc, vw, opt, outs) if save_for_backward else None
+        self._last = (list(scenes), sc, vw, outs) if save_for_backward else None
         return outs
 
     def stats(self) -> dict:
@@ -161,23 +192,27 @@ class Renderer:
         return out
 
     # -- backward -----------------------------------------------------------------------
-    def backward(self, d_depth: Sequence[Optional[torch.Tensor]], d_alpha: Sequence[Optional[torch.Tensor]]):
+    def backward(self, d_depth: Sequence[Optional[torch.Tensor]], d_alpha: Sequence[Optional[torch.Tensor]],
+                 grads=None, reuse_descriptors: bool = False):
         """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> with respect to every
         scene's (means, prec upper triangle, weights_log), from the last forward with
-        save_for_backward=True."""
+        save_for_backward=True.  ``grads`` (optional) = [(d_means, d_prec, d_w), ...] buffers
+        to accumulate into; by default fresh zeroed tensors are returned."""
         if self._last is None:
             raise FeError("backward: the last forward was not run with save_for_backward=True")
-        scenes, sc, vw, opt, outs = self._last
+        scenes, sc, vw, outs = self._last
         cot = (FeViewCotangent * len(outs))()
-        keep = []
         for i in range(len(outs)):
-            dd = None if d_depth[i] is None else d_depth[i].to(torch.float32).contiguous()
-            da = None if d_alpha[i] is None else d_alpha[i].to(torch.float32).contiguous()
-            keep += [dd, da]
+            dd, da = d_depth[i], d_alpha[i]
+            for t in (dd, da):
+                if t is not None and (t.dtype != torch.float32 or not t.is_contiguous() or t.device != self.device):
+                    raise FeError("backward: cotangents must be contiguous float32 tensors on the render device")
             cot[i] = FeViewCotangent(_ptr(dd), _ptr(da))
-        grads = [(torch.zeros_like(s.means), torch.zeros_like(s.prec), torch.zeros_like(s.weights_log))
-                 for s in scenes]
+        if grads is None:
+            grads = [(torch.zeros_like(s.means), torch.zeros_like(s.prec), torch.zeros_like(s.weights_log))
+                     for s in scenes]
         gs = (FeSceneGrad * len(scenes))(*[FeSceneGrad(_ptr(a), _ptr(b), _ptr(c)) for a, b, c in grads])
+        opt = self._opt(True, reuse_descriptors)
         ws = self._ws
         _lib.check(self.lib.fe_backward(_stream(self.device), sc, len(scenes), vw, len(outs), ctypes.byref(opt),
                                         cot, gs, ws.data_ptr(), ws.numel()), "fe_backward")

@@ -37,7 +37,8 @@ class FeScene(ctypes.Structure):
 
 class FeView(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_int32), ("cam", FeCamera), ("beta2", ctypes.c_float),
-                ("beta3", ctypes.c_float), ("depth", ctypes.c_void_p), ("alpha", ctypes.c_void_p)]
+                ("beta3", ctypes.c_float), ("depth", ctypes.c_void_p), ("alpha", ctypes.c_void_p),
+                ("pose_device", ctypes.c_void_p)]
 
 
 class FeSceneGrad(ctypes.Structure):
@@ -51,7 +52,10 @@ class FeViewCotangent(ctypes.Structure):
 
 class FeOptions(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int32), ("batch", ctypes.c_int32), ("mode", ctypes.c_int32),
-                ("save_for_backward", ctypes.c_int32)]
+                ("save_for_backward", ctypes.c_int32), ("flags", ctypes.c_int32)]
+
+
+FE_FLAG_REUSE_DESCRIPTORS = 1
 
 
 class FeStats(ctypes.Structure):

@@ -12,8 +12,9 @@
 //   g_q = g_d2 v + (g_t/vsv) u
 //   g_u = -g_d2 t v + (g_t/vsv) (q - 2 t u)
 //   g_w = g_d2
-// Per (Gaussian, view) the kernel accumulates 13 sums: G_q (3), G_a = sum g_u x' (3),
-// G_b = sum g_u y' (3), G_g = sum g_u (3), G_w; k_bwd_finalize maps them to d(mu), d(U), d(w).
+// Per (tile, Gaussian) the kernel accumulates 13 sums: G_q (3), G_a = sum g_u x' (3),
+// G_b = sum g_u y' (3), G_g = sum g_u (3), G_w, then maps them through the chain rule to the
+// 10 gradient components of (mu, upper U, w) and adds those to the scene gradients.
 #include "fe_common.h"
 
 #include <string.h>
@@ -21,11 +22,6 @@
 #include <algorithm>
 
 namespace fe {
-
-__global__ void k_zero(float* __restrict__ p, int64_t n) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-    p[i] = 0.0f;
-}
 
 struct BwdArgs {
   const Job* jobs;
@@ -37,7 +33,6 @@ struct BwdArgs {
   const uint32_t* list;
   const float* st;
   int64_t total_pix;
-  float* gacc;   // 16 floats per (Gaussian, view)
   int dense;
 };
 
@@ -151,48 +146,40 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
       Gg0 += gu0; Gg1 += gu1; Gg2 += gu2;
       Gw += gd2;
     }
-    float* G = A.gacc + (jb.rec_base + gid) * 16;
-    atomicAdd(G + 0, Gq0); atomicAdd(G + 1, Gq1); atomicAdd(G + 2, Gq2);
-    atomicAdd(G + 3, Ga0); atomicAdd(G + 4, Ga1); atomicAdd(G + 5, Ga2);
-    atomicAdd(G + 6, Gb0); atomicAdd(G + 7, Gb1); atomicAdd(G + 8, Gb2);
-    atomicAdd(G + 9, Gg0); atomicAdd(G + 10, Gg1); atomicAdd(G + 11, Gg2);
-    atomicAdd(G + 12, Gw);
-  }
-}
-
-// Per (Gaussian, view): chain rule from the 13 sums to d(mu), d(U upper), d(w).
-//   M = A R^T, a = M[:,0]/fx, b = M[:,1]/fy, g = M[:,2], q = A (mu - o), A = U^T.
-__global__ __launch_bounds__(256) void k_bwd_finalize(const Job* __restrict__ jobs,
-                                                      const BwdJob* __restrict__ bjobs,
-                                                      const float* __restrict__ gacc) {
-  const Job& jb = jobs[blockIdx.y];
-  const BwdJob& bj = bjobs[blockIdx.y];
-  const int j = blockIdx.x * 256 + threadIdx.x;
-  if (j >= jb.n) return;
-  const float* G = gacc + (jb.rec_base + j) * 16;
-  const float* P = jb.prec + 9 * (int64_t)j;
-  const float* mu = jb.means + 3 * (int64_t)j;
-  const float A[3][3] = {{P[0], 0.f, 0.f}, {P[1], P[4], 0.f}, {P[2], P[5], P[8]}};
-  const float p[3] = {mu[0] - jb.o[0], mu[1] - jb.o[1], mu[2] - jb.o[2]};
-  float dM[3][3];
-  for (int i = 0; i < 3; ++i) {
-    dM[i][0] = G[3 + i] / jb.fx;
-    dM[i][1] = G[6 + i] / jb.fy;
-    dM[i][2] = G[9 + i];
-  }
-  float dA[3][3];
-  for (int i = 0; i < 3; ++i)
-    for (int k = 0; k < 3; ++k)
-      dA[i][k] = dM[i][0] * jb.R[0 * 3 + k] + dM[i][1] * jb.R[1 * 3 + k] + dM[i][2] * jb.R[2 * 3 + k] +
-                 G[i] * p[k];
-  float dmu[3];
-  for (int k = 0; k < 3; ++k) dmu[k] = A[0][k] * G[0] + A[1][k] * G[1] + A[2][k] * G[2];
-  if (bj.dmeans)
-    for (int k = 0; k < 3; ++k) atomicAdd(bj.dmeans + 3 * (int64_t)j + k, dmu[k]);
-  if (bj.dprec)
+    // chain rule for this (tile, Gaussian) pair, straight into the scene gradients:
+    //   dM = [G_a/fx, G_b/fy, G_g],  dA = dM R + G_q p^T (lower part),  dmu = A^T G_q,
+    //   d(prec)[k][i] += dA[i][k] for k <= i,  dw += G_w
+    const float* P = jb.prec + 9 * gid;
+    const float* mu = jb.means + 3 * gid;
+    const float u00 = P[0], u01 = P[1], u02 = P[2], u11 = P[4], u12 = P[5], u22 = P[8];
+    const float p0 = mu[0] - jb.o[0], p1 = mu[1] - jb.o[1], p2 = mu[2] - jb.o[2];
+    const float ifx = 1.0f / jb.fx, ify = 1.0f / jb.fy;
+    const float dm[3][3] = {{Ga0 * ifx, Gb0 * ify, Gg0}, {Ga1 * ifx, Gb1 * ify, Gg1}, {Ga2 * ifx, Gb2 * ify, Gg2}};
+    const float gq[3] = {Gq0, Gq1, Gq2};
+    const float pv[3] = {p0, p1, p2};
+    float dA[3][3];
+#pragma unroll
     for (int i = 0; i < 3; ++i)
-      for (int k = 0; k <= i; ++k) atomicAdd(bj.dprec + 9 * (int64_t)j + 3 * k + i, dA[i][k]);  // U[k][i] = A[i][k]
-  if (bj.dw) atomicAdd(bj.dw + j, G[12]);
+#pragma unroll
+      for (int k = 0; k <= i; ++k)
+        dA[i][k] = fmaf(dm[i][0], jb.R[k], fmaf(dm[i][1], jb.R[3 + k], fmaf(dm[i][2], jb.R[6 + k], gq[i] * pv[k])));
+    if (bj.dmeans) {
+      float* dmu = bj.dmeans + 3 * gid;
+      atomicAdd(dmu + 0, fmaf(u00, Gq0, fmaf(u01, Gq1, u02 * Gq2)));
+      atomicAdd(dmu + 1, fmaf(u11, Gq1, u12 * Gq2));
+      atomicAdd(dmu + 2, u22 * Gq2);
+    }
+    if (bj.dprec) {
+      float* dp = bj.dprec + 9 * gid;
+      atomicAdd(dp + 0, dA[0][0]);
+      atomicAdd(dp + 1, dA[1][0]);
+      atomicAdd(dp + 2, dA[2][0]);
+      atomicAdd(dp + 4, dA[1][1]);
+      atomicAdd(dp + 5, dA[2][1]);
+      atomicAdd(dp + 8, dA[2][2]);
+    }
+    if (bj.dw) atomicAdd(bj.dw + gid, Gw);
+  }
 }
 
 }  // namespace fe
@@ -201,13 +188,6 @@ __global__ __launch_bounds__(256) void k_bwd_finalize(const Job* __restrict__ jo
 // launchers
 // ------------------------------------------------------------------------------------------
 namespace fe {
-
-hipError_t launch_zero(hipStream_t s, float* p, int64_t n) {
-  if (n <= 0) return hipSuccess;
-  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_zero, dim3((unsigned)blocks), dim3(256), 0, s, p, n);
-  return hipGetLastError();
-}
 
 template <int TS>
 static hipError_t bwd_t(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
@@ -218,8 +198,8 @@ static hipError_t bwd_t(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int gb, int64_t total_tiles, const Job* jobs,
                             const void* bjobs, int n_jobs, const float* rec, const unsigned long long* off1,
                             const unsigned long long* off2, const uint32_t* list, const float* st,
-                            int64_t total_pix, float* gacc, int dense) {
-  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, off1, off2, list, st, total_pix, gacc, dense};
+                            int64_t total_pix, int dense) {
+  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, off1, off2, list, st, total_pix, dense};
   (void)gb;
   switch (ts) {
     case 8: return bwd_t<8>(s, total_tiles, a);
@@ -227,14 +207,6 @@ hipError_t launch_bwd_blend(hipStream_t s, int ts, int gb, int64_t total_tiles, 
     case 32: return bwd_t<32>(s, total_tiles, a);
   }
   return hipErrorInvalidValue;
-}
-
-hipError_t launch_bwd_finalize(hipStream_t s, const Job* jobs, const void* bjobs, int n_jobs, int max_n,
-                               const float* gacc) {
-  if (max_n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_bwd_finalize, dim3((max_n + 255) / 256, n_jobs), dim3(256), 0, s, jobs,
-                     reinterpret_cast<const BwdJob*>(bjobs), gacc);
-  return hipGetLastError();
 }
 
 size_t bwd_job_bytes() { return sizeof(BwdJob); }

@@ -41,12 +41,12 @@ static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 struct Layout {
   int n_jobs = 0;
   int max_n = 0;
-  int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0;
+  int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0, reuse = 0;
   int64_t total_stiles = 0;
   int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
-         o_gacc = 0, o_bjobs = 0, bytes = 0;
+         o_bjobs = 0, bytes = 0;
 };
 
 // A small header at the start of the workspace lets fe_read_stats find the counters.
@@ -63,7 +63,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     set_msg("fe_plan: need at least one scene and one view");
     return FE_ERR_INVALID_ARGUMENT;
   }
-  fe_options o = {16, 64, FE_MODE_CULLED, 0};
+  fe_options o = {16, 64, FE_MODE_CULLED, 0, 0};
   if (opt) o = *opt;
   if (o.tile == 0) o.tile = 16;
   if (o.batch == 0) o.batch = 64;
@@ -83,6 +83,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->gb = o.batch;
   L->mode = o.mode;
   L->save = o.save_for_backward ? 1 : 0;
+  L->reuse = (o.flags & FE_FLAG_REUSE_DESCRIPTORS) ? 1 : 0;
   L->n_jobs = n_views;
   if (jobs) jobs->resize(n_views);
   int64_t rec = 0, tiles = 0, stiles = 0, maskw = 0, pix = 0, cap = 0;
@@ -115,6 +116,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       jb.wlog = sc.weights_log;
       jb.depth = vw.depth;
       jb.alpha = vw.alpha;
+      jb.pose = vw.pose_device;
       memcpy(jb.R, c.R, sizeof(jb.R));
       memcpy(jb.o, c.t, sizeof(jb.o));
       jb.fx = c.fx; jb.fy = c.fy; jb.cx = c.cx; jb.cy = c.cy;
@@ -176,29 +178,35 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     L->o_list = p; p = align256(p + sizeof(uint32_t) * (size_t)std::max<int64_t>(cap, 1));
   }
   if (L->save) {
-    L->o_gacc = p; p = align256(p + sizeof(float) * 16 * (size_t)std::max<int64_t>(rec, 1));
     L->o_bjobs = p; p = align256(p + sizeof(BwdJob) * n_views);
   }
   L->bytes = p;
   return FE_OK;
 }
 
-// Pinned staging for the job descriptors, one per device.  Before overwriting, wait for the
-// previous upload from it (normally long finished).
-struct Staging {
-  std::mutex mu;
+// Pinned staging for descriptor uploads: a ring of slots per device.  Before a slot is
+// overwritten, the upload that last used it must have completed (normally long ago).
+struct Slot {
   void* host = nullptr;
   size_t cap = 0;
   hipEvent_t ev = nullptr;
   bool pending = false;
+};
+constexpr int kSlots = 8;
+struct Staging {
+  std::mutex mu;
+  Slot slot[kSlots];
+  int next = 0;
 };
 static Staging g_staging[64];
 
 static fe_status upload(hipStream_t s, void* dst, const void* src, size_t bytes) {
   int dev = 0;
   FE_HIP(hipGetDevice(&dev));
-  Staging& st = g_staging[dev & 63];
-  std::lock_guard<std::mutex> lk(st.mu);
+  Staging& stg = g_staging[dev & 63];
+  std::lock_guard<std::mutex> lk(stg.mu);
+  Slot& st = stg.slot[stg.next];
+  stg.next = (stg.next + 1) % kSlots;
   if (st.pending) {
     FE_HIP(hipEventSynchronize(st.ev));
     st.pending = false;
@@ -289,8 +297,10 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
              L.o_rec, L.o_list, L.o_lmin, (uint64_t)L.total_rec, L.o_state, (uint64_t)L.total_pix};
   memcpy(blob.data() + L.o_header, &h, sizeof(h));
   memcpy(blob.data() + L.o_jobs, jobs.data(), sizeof(Job) * n_views);
-  st = upload(s, ws, blob.data(), blob.size());
-  if (st != FE_OK) return st;
+  if (!L.reuse) {
+    st = upload(s, ws, blob.data(), blob.size());
+    if (st != FE_OK) return st;
+  }
 
   const Job* dj = at<Job>(ws, L.o_jobs);
   Counters* cnt = at<Counters>(ws, L.o_counters);
@@ -300,6 +310,9 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   float* lmin = at<float>(ws, L.o_lmin);
   float* ubm = at<float>(ws, L.o_ubm);
   FE_HIP(launch_init(s, cnt, flags, L.n_jobs));
+  bool any_pose = false;
+  for (int v = 0; v < n_views; ++v) any_pose = any_pose || views[v].pose_device != nullptr;
+  if (any_pose) FE_HIP(launch_pose(s, at<Job>(ws, L.o_jobs), L.n_jobs));
   FE_HIP(timed(s, FE_K_PREPROCESS, [&] { return launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, flags); }));
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
@@ -345,7 +358,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
 fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
                    int32_t n_views, const fe_options* opt, const fe_view_cotangent* cot,
                    const fe_scene_grad* grads, void* ws, size_t ws_bytes) {
-  fe_options o = {16, 64, FE_MODE_CULLED, 1};
+  fe_options o = {16, 64, FE_MODE_CULLED, 1, 0};
   if (opt) o = *opt;
   o.save_for_backward = 1;
   Layout L;
@@ -369,21 +382,20 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
     bj[v].dprec = g.d_prec;
     bj[v].dw = g.d_weights_log;
   }
-  st = upload(s, at<char>(ws, L.o_bjobs), bj.data(), sizeof(BwdJob) * n_views);
-  if (st != FE_OK) return st;
+  if (!L.reuse) {
+    st = upload(s, at<char>(ws, L.o_bjobs), bj.data(), sizeof(BwdJob) * n_views);
+    if (st != FE_OK) return st;
+  }
   const Job* dj = at<Job>(ws, L.o_jobs);
   const void* dbj = at<char>(ws, L.o_bjobs);
-  float* gacc = at<float>(ws, L.o_gacc);
-  FE_HIP(launch_zero(s, gacc, 16 * L.total_rec));
   const bool dense = (L.mode == FE_MODE_DENSE);
   FE_HIP(timed(s, FE_K_BWD_BLEND, [&] {
     return launch_bwd_blend(s, L.ts, L.gb, L.total_tiles, dj, dbj, L.n_jobs, at<float>(ws, L.o_rec),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off1),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off2),
                             dense ? nullptr : at<uint32_t>(ws, L.o_list), at<float>(ws, L.o_state), L.total_pix,
-                            gacc, dense ? 1 : 0);
+                            dense ? 1 : 0);
   }));
-  FE_HIP(timed(s, FE_K_BWD_FINALIZE, [&] { return launch_bwd_finalize(s, dj, dbj, L.n_jobs, L.max_n, gacc); }));
   return FE_OK;
 }
 
@@ -493,7 +505,7 @@ fe_status fe_render_workspace_bytes(int32_t n, int32_t height, int32_t width, in
   v.cam.height = height;
   v.cam.width = width;
   v.cam.fx = v.cam.fy = 1.0f;
-  fe_options o{tile, 64, FE_MODE_CULLED, 0};
+  fe_options o{tile, 64, FE_MODE_CULLED, 0, 0};
   return fe_plan(&sc, 1, &v, 1, &o, bytes);
 }
 
@@ -509,7 +521,7 @@ fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float
   v.beta3 = beta3;
   v.depth = depth;
   v.alpha = alpha;
-  fe_options o{16, 64, FE_MODE_CULLED, 0};
+  fe_options o{16, 64, FE_MODE_CULLED, 0, 0};
   return fe_forward(stream, &sc, 1, &v, 1, &o, workspace, workspace_bytes);
 }
 

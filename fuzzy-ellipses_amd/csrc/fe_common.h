@@ -30,6 +30,7 @@ struct Job {
   const float* wlog;
   float* depth;
   float* alpha;
+  const float* pose;  // optional device pose R[9] t[3], copied into R / o by k_pose
   float R[9];
   float o[3];
   float fx, fy, cx, cy;

@@ -54,6 +54,18 @@ __global__ void k_init(Counters* cnt, int32_t* job_flags, int n_jobs) {
 }
 
 // ------------------------------------------------------------------------------------------
+// k_pose: jobs whose pose lives in device memory get R and o copied into their descriptor
+// (one thread per job; runs before every kernel that reads the pose)
+// ------------------------------------------------------------------------------------------
+__global__ void k_pose(Job* jobs, int n_jobs) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_jobs || jobs[j].pose == nullptr) return;
+  const float* p = jobs[j].pose;
+  for (int i = 0; i < 9; ++i) jobs[j].R[i] = p[i];
+  for (int i = 0; i < 3; ++i) jobs[j].o[i] = p[9 + i];
+}
+
+// ------------------------------------------------------------------------------------------
 // (a) preprocess: grid (ceil(max_n/256), n_jobs), one thread per Gaussian of the job.
 //     Reads means (N,3), prec (N,3,3) (upper triangle), w (N,) -- a wave reads one
 //     contiguous span of each array.
@@ -586,6 +598,11 @@ namespace fe {
 hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs) {
   const int blocks = max(1, min(64, (n_jobs + 255) / 256));
   hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, s, cnt, job_flags, n_jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_pose(hipStream_t s, Job* jobs, int n_jobs) {
+  hipLaunchKernelGGL(k_pose, dim3((n_jobs + 63) / 64), dim3(64), 0, s, jobs, n_jobs);
   return hipGetLastError();
 }
 

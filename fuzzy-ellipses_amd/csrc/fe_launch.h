@@ -10,6 +10,7 @@ struct Job;
 struct Counters;
 
 hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs);
+hipError_t launch_pose(hipStream_t s, Job* jobs, int n_jobs);
 hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec,
                              int32_t* job_flags);
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
@@ -27,12 +28,9 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
                         float* tile_lmin, const float* tile_ubm, Counters* cnt, int save);
 
-hipError_t launch_zero(hipStream_t s, float* p, int64_t n);
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int gb, int64_t total_tiles, const Job* jobs,
                             const void* bjobs, int n_jobs, const float* rec, const unsigned long long* off1,
                             const unsigned long long* off2, const uint32_t* list, const float* st,
-                            int64_t total_pix, float* gacc, int dense);
-hipError_t launch_bwd_finalize(hipStream_t s, const Job* jobs, const void* bjobs, int n_jobs, int max_n,
-                               const float* gacc);
+                            int64_t total_pix, int dense);
 
 }  // namespace fe

@@ -19,7 +19,7 @@ from __future__ import annotations
 
 import dataclasses
 import math
-from typing import List
+from typing import List, Optional
 
 import numpy as np
 
@@ -202,15 +202,22 @@ def batched_scenes(seed: int = 2, n_scenes: int = 256, n_lo: int = 16, n_hi: int
 
 
 def silhouette_problem(seed: int = 3, n: int = 1024, n_views: int = 16, H: int = 240, W: int = 320,
-                       f: float = 300.0, radius: float = 3.0):
+                       f: float = 300.0, radius: float = 3.0, weight_shift: Optional[float] = None):
     """Shape-from-silhouette workload: an initial scene and a target scene (seed+1) sharing
-    16 fibonacci views at 240x320.  Focal length f = W * 600/640."""
+    16 fibonacci views at 240x320, focal length f = W * 600/640.
+
+    With log-softmaxed weights a mixture's alpha stays near sum(exp(w)) / n (about 0.02 here),
+    so no pixel would pass the 0.5 silhouette threshold; both scenes' log-weights are
+    therefore shifted by `weight_shift` (default ln n: weights N(0, 0.5) before
+    normalisation), which makes the silhouettes opaque as in the reference notebook's fits.
+    """
+    shift = math.log(n) if weight_shift is None else float(weight_shift)
     rng = np.random.default_rng(seed)
     means, prec, w = random_gaussians(rng, n)
     rng_t = np.random.default_rng(seed + 1)
     tm, tp, tw = random_gaussians(rng_t, n)
     cams = [make_camera(radius * d, H, W, f) for d in fibonacci_dirs(n_views)]
-    return Scene(means, prec, w, cams), Scene(tm, tp, tw, cams)
+    return Scene(means, prec, w + shift, cams), Scene(tm, tp, tw + shift, cams)
 
 
 SERVE_RESOLUTIONS = [(128, 128), (240, 320), (480, 640)]

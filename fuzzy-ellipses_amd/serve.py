@@ -1,0 +1,232 @@
+"""Continuous-batching render server.
+
+Scenes' Gaussians stay resident in HBM.  Requests (render, or render + gradient with a
+given cotangent) arrive over time; the loop admits queued requests into a batch while the
+HBM budget (resident scenes + in-flight workspaces, outputs and gradient buffers) allows,
+launches the batch on a free stream slot and goes back to admitting while it runs, so
+requests that arrive while others are on the GPU join the next batch and overlap with the
+running one.  Each request in a batch is its own (scene, view) job, so its outputs and
+gradients are its own; the same request alone or in a mix gives the same result.
+
+The plain path is the same loop with one request per batch and one slot.
+"""
+from __future__ import annotations
+
+import collections
+import dataclasses
+import math
+import time
+from typing import Callable, Deque, Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import Renderer, SceneTensors
+from .scenes import Camera, Scene
+
+GB = 1 << 30
+
+
+@dataclasses.dataclass
+class Request:
+    rid: int
+    scene: int
+    camera: Camera
+    backward: bool
+    arrival: float                       # seconds after the start of the run
+    d_depth: Optional[torch.Tensor] = None
+    d_alpha: Optional[torch.Tensor] = None
+    beta2: float = 21.4
+    beta3: float = 2.66
+    # filled in by the server
+    depth: Optional[torch.Tensor] = None
+    alpha: Optional[torch.Tensor] = None
+    grads: Optional[Tuple[torch.Tensor, torch.Tensor, torch.Tensor]] = None
+    t_launch: float = math.nan
+    t_done: float = math.nan
+
+    @property
+    def latency_ms(self) -> float:
+        return 1e3 * (self.t_done - self.arrival)
+
+
+class Scheduler:
+    """Continuous-batching loop, independent of the device.
+
+    `launch(slot, batch)` starts a batch on slot `slot` and returns a handle;
+    `ready(handle)` says whether it finished; `bytes_of(request)` is the memory a request
+    holds while in flight.  Requests are admitted FIFO into a batch for the first idle slot
+    while the free budget allows (at most `max_batch` per batch); a request that cannot fit
+    waits for in-flight batches to retire.  `clock()` returns seconds; `sleep(dt)` waits.
+    """
+
+    def __init__(self, n_slots: int, max_batch: int, free_bytes: int, launch, ready, bytes_of,
+                 clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep):
+        self.n_slots = max(1, int(n_slots))
+        self.max_batch = max(1, int(max_batch))
+        self.free_bytes = int(free_bytes)
+        self.launch, self.ready, self.bytes_of = launch, ready, bytes_of
+        self.clock, self.sleep = clock, sleep
+        self.peak_in_flight = 0
+        self.batch_sizes: List[int] = []
+
+    def run(self, requests: Sequence[Request]) -> List[Request]:
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        queue: Deque[Request] = collections.deque()
+        slots: List[Optional[Tuple[object, List[Request], int]]] = [None] * self.n_slots
+        done: List[Request] = []
+        in_flight = 0
+        i = 0
+        t0 = self.clock()
+        while len(done) < len(reqs):
+            now = self.clock() - t0
+            while i < len(reqs) and reqs[i].arrival <= now:
+                queue.append(reqs[i])
+                i += 1
+            progressed = False
+            for k, sl in enumerate(slots):
+                if sl is not None and self.ready(sl[0]):
+                    t = self.clock() - t0
+                    for r in sl[1]:
+                        r.t_done = t
+                    done += sl[1]
+                    in_flight -= sl[2]
+                    slots[k] = None
+                    progressed = True
+            idle = [k for k, sl in enumerate(slots) if sl is None]
+            if queue and idle:
+                batch: List[Request] = []
+                nbytes = 0
+                while queue and len(batch) < self.max_batch:
+                    b = self.bytes_of(queue[0])
+                    if in_flight + nbytes + b > self.free_bytes:
+                        break
+                    batch.append(queue.popleft())
+                    nbytes += b
+                if batch:
+                    k = idle[0]
+                    t = self.clock() - t0
+                    for r in batch:
+                        r.t_launch = t
+                    slots[k] = (self.launch(k, batch), batch, nbytes)
+                    in_flight += nbytes
+                    self.peak_in_flight = max(self.peak_in_flight, in_flight)
+                    self.batch_sizes.append(len(batch))
+                    progressed = True
+                elif len(idle) == self.n_slots:
+                    raise MemoryError(f"request {queue[0].rid} needs more than the free memory budget")
+            if not progressed:
+                busy = any(sl is not None for sl in slots)
+                if busy:
+                    self.sleep(2e-5)
+                elif i < len(reqs):
+                    self.sleep(max(0.0, min(1e-3, reqs[i].arrival - (self.clock() - t0))))
+        return done
+
+
+class _Slot:
+    """One in-flight batch position: a stream and two renderers (fwd-only / fwd+bwd)."""
+
+    def __init__(self, device, tile, batch):
+        self.stream = torch.cuda.Stream(device=device)
+        self.fwd = Renderer(tile=tile, batch=batch, mode="culled", device=device)
+        self.fb = Renderer(tile=tile, batch=batch, mode="culled", device=device)
+
+
+class Server:
+    """Resident scenes + continuous batching on `slots` streams under an HBM budget."""
+
+    def __init__(self, scenes: Sequence[Scene], device="cuda", budget_bytes: int = 64 * GB,
+                 max_batch: int = 64, slots: int = 2, tile: int = 16, batch: int = 64,
+                 clock: Callable[[], float] = time.perf_counter):
+        self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
+        self.scenes = [SceneTensors.from_scene(s, self.device) for s in scenes]
+        self.scene_bytes = sum(s.n * 52 for s in self.scenes)
+        self.budget = int(budget_bytes)
+        if self.scene_bytes > self.budget:
+            raise ValueError("resident scenes exceed the memory budget")
+        self.max_batch = int(max_batch)
+        self.tile = tile
+        self.slots = [_Slot(self.device, tile, batch) for _ in range(max(1, slots))]
+        self.clock = clock
+
+    def request_bytes(self, req: Request) -> int:
+        """Bytes a request holds while in flight: its share of the workspace (tile lists sized
+        for the worst case, records, masks, per-pixel state), outputs and gradient buffers."""
+        n = self.scenes[req.scene].n
+        cam = req.camera
+        tiles = math.ceil(cam.W / self.tile) * math.ceil(cam.H / self.tile)
+        b = 4 * n * tiles + 64 * n + 24 * cam.H * cam.W + 16 * tiles * math.ceil(n / 64) + 32 * tiles
+        b += 8 * cam.H * cam.W                      # depth + alpha
+        if req.backward:
+            b += 52 * n + 8 * cam.H * cam.W         # gradients + cotangents
+        return b + 4096
+
+    def _launch(self, k: int, batch: List[Request]):
+        slot = self.slots[k]
+        with torch.cuda.stream(slot.stream):
+            groups = ([r for r in batch if not r.backward], [r for r in batch if r.backward])
+            for g, rnd in zip(groups, (slot.fwd, slot.fb)):
+                if not g:
+                    continue
+                scenes = [self.scenes[r.scene] for r in g]
+                views = [(i, r.camera, r.beta2, r.beta3) for i, r in enumerate(g)]
+                outs = rnd.alloc_outputs(views)
+                rnd.forward(scenes, views, save_for_backward=(rnd is slot.fb), outs=outs)
+                for r, (d, a) in zip(g, outs):
+                    r.depth, r.alpha = d, a
+                if rnd is slot.fb:
+                    grads = rnd.backward([r.d_depth for r in g], [r.d_alpha for r in g])
+                    for r, gr in zip(g, grads):
+                        r.grads = gr
+            ev = torch.cuda.Event()
+            ev.record(slot.stream)
+        return ev
+
+    def run(self, requests: Sequence[Request]) -> Dict[str, float]:
+        torch.cuda.synchronize(self.device)      # inputs (cotangents) are ready
+        sch = Scheduler(len(self.slots), self.max_batch, self.budget - self.scene_bytes, self._launch,
+                        lambda ev: ev.query(), self.request_bytes, clock=self.clock)
+        t0 = self.clock()
+        done = sch.run(requests)
+        total = self.clock() - t0
+        lat = np.array([r.latency_ms for r in done])
+        pix = sum(r.camera.H * r.camera.W for r in done)
+        return {"requests": len(done), "batches": len(sch.batch_sizes), "seconds": total,
+                "mean_batch": float(np.mean(sch.batch_sizes)) if sch.batch_sizes else 0.0,
+                "throughput_rps": len(done) / total, "megapixels_per_s": pix / total / 1e6,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "mean_ms": float(lat.mean()), "peak_in_flight_bytes": sch.peak_in_flight}
+
+
+# ------------------------------------------------------------------------------------------
+# synthetic request streams
+# ------------------------------------------------------------------------------------------
+def poisson_requests(scenes: Sequence[Scene], rate: float, n: int, seed: int = 4, backward_frac: float = 0.3,
+                     resolutions=((128, 128), (240, 320), (480, 640)), device="cuda",
+                     burst: bool = False) -> List[Request]:
+    """Poisson arrivals at `rate` req/s (all at t=0 if `burst`), scene uniform, resolution
+    uniform over `resolutions`, camera on a random direction at radius U[2.5, 3.5] looking
+    at the origin (f = W * 600/640), a `backward_frac` share asking for gradients with a
+    random cotangent (one shared cotangent per resolution)."""
+    from .scenes import make_camera, random_dir
+    rng = np.random.default_rng(seed)
+    dev = torch.device(device)
+    cot = {}
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    for (H, W) in resolutions:
+        cot[(H, W)] = (torch.randn(H, W, generator=g).to(dev), torch.randn(H, W, generator=g).to(dev))
+    t = 0.0
+    out = []
+    for k in range(n):
+        if not burst:
+            t += float(rng.exponential(1.0 / rate))
+        si = int(rng.integers(len(scenes)))
+        H, W = resolutions[int(rng.integers(len(resolutions)))]
+        cam = make_camera(float(rng.uniform(2.5, 3.5)) * random_dir(rng), H, W, W * 600.0 / 640.0)
+        bwd = bool(rng.random() < backward_frac)
+        d_d, d_a = cot[(H, W)] if bwd else (None, None)
+        out.append(Request(k, si, cam, bwd, t, d_d, d_a, scenes[si].beta2, scenes[si].beta3))
+    return out

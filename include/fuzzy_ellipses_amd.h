@@ -69,6 +69,9 @@ typedef struct fe_view {
   float beta2, beta3;
   float* depth;    /* (H,W) device, written by fe_forward */
   float* alpha;    /* (H,W) device, written by fe_forward */
+  const float* pose_device; /* optional device array R[9] (row-major) then t[3]; when set it
+                               replaces cam.R / cam.t, read on the device (the reference
+                               passed R and t as device buffers, rasterize_main.cu:60-61) */
 } fe_view;
 
 /* Gradient buffers of one scene (device, accumulated into: zero them first). */
@@ -89,11 +92,20 @@ enum {
   FE_MODE_DENSE = 1    /* every tile evaluates every Gaussian */
 };
 
+enum {
+  /* The workspace already holds this call's descriptors, written by an earlier call with
+   * the same scenes, views, options and buffers: skip the host-to-device upload.  With this
+   * flag (and profiling off) a call only enqueues kernels, so it can be captured into a
+   * hipGraph and replayed. */
+  FE_FLAG_REUSE_DESCRIPTORS = 1
+};
+
 typedef struct fe_options {
   int32_t tile;        /* tile edge in pixels: 8, 16 or 32 (0 = 16) */
   int32_t batch;       /* LDS batch depth in Gaussians: 32, 64, 128 or 256 (0 = 64) */
   int32_t mode;        /* FE_MODE_* */
   int32_t save_for_backward; /* keep per-pixel aggregates for fe_backward */
+  int32_t flags;       /* FE_FLAG_* */
 } fe_options;
 
 /* Diagnostics written by fe_forward into the workspace and readable with fe_read_stats. */

@@ -38,7 +38,7 @@ def _one_view(fea, H=480, W=640, n=4096, tile=16, batch=64, mode=0, save=0):
     cam.cx, cam.cy = W / 2, H / 2
     cam.height, cam.width = H, W
     vw = (_lib.FeView * 1)(_lib.FeView(0, cam, 21.4, 2.66, None, None))
-    opt = _lib.FeOptions(tile, batch, mode, save)
+    opt = _lib.FeOptions(tile, batch, mode, save, 0)
     nbytes = ctypes.c_size_t(0)
     st = _lib.load().fe_plan(sc, 1, vw, 1, ctypes.byref(opt), ctypes.byref(nbytes))
     return st, nbytes.value
@@ -78,7 +78,7 @@ def test_forward_rejects_small_workspace(fea):
     cam.fx = cam.fy = 8.0
     cam.height = cam.width = 8
     vw = (_lib.FeView * 1)(_lib.FeView(0, cam, 21.4, 2.66, None, None))
-    opt = _lib.FeOptions(8, 32, 0, 0)
+    opt = _lib.FeOptions(8, 32, 0, 0, 0)
     st = lib.fe_forward(None, sc, 1, vw, 1, ctypes.byref(opt), None, 0)
     assert st == 2  # FE_ERR_WORKSPACE
 

@@ -1,0 +1,95 @@
+"""Continuous-batching scheduler (fuzzy-ellipses_amd/serve.py) on a fake device and clock."""
+import numpy as np
+import pytest
+
+
+class FakeDevice:
+    """Batches take `per_req` seconds per request of simulated time."""
+
+    def __init__(self, clock, per_req=0.01):
+        self.clock, self.per_req = clock, per_req
+        self.launched = []
+
+    def launch(self, slot, batch):
+        self.launched.append((slot, [r.rid for r in batch], self.clock.t))
+        return self.clock.t + self.per_req * len(batch)
+
+    def ready(self, handle):
+        return self.clock.t >= handle
+
+
+class FakeClock:
+    def __init__(self):
+        self.t = 0.0
+
+    def __call__(self):
+        return self.t
+
+    def sleep(self, dt):
+        self.t += max(dt, 1e-4)
+
+
+def _reqs(fea, n, rate=None, bwd_every=3):
+    from fuzzy_ellipses_amd.serve import Request
+    import fuzzy_ellipses_amd.scenes as sc
+    cam = sc.make_camera([0, 0, 3.0], 8, 8, 8.0)
+    rng = np.random.default_rng(0)
+    t = 0.0
+    out = []
+    for k in range(n):
+        if rate:
+            t += rng.exponential(1 / rate)
+        out.append(Request(k, k % 3, cam, k % bwd_every == 0, t))
+    return out
+
+
+def _sched(fea, clock, dev, slots=2, max_batch=8, free=10 ** 9, bytes_of=lambda r: 100):
+    from fuzzy_ellipses_amd.serve import Scheduler
+    return Scheduler(slots, max_batch, free, dev.launch, dev.ready, bytes_of, clock=clock, sleep=clock.sleep)
+
+
+def test_every_request_completes_and_batches_form_under_load(fea):
+    clock = FakeClock()
+    dev = FakeDevice(clock, per_req=0.002)
+    reqs = _reqs(fea, 200, rate=2000)
+    done = _sched(fea, clock, dev).run(reqs)
+    assert sorted(r.rid for r in done) == list(range(200))
+    assert all(r.t_done >= r.t_launch >= r.arrival for r in done)
+    sizes = [len(b) for _, b, _ in dev.launched]
+    assert max(sizes) > 1          # arrivals during a running batch were batched together
+    assert max(sizes) <= 8
+
+
+def test_continuous_admission_overlaps_slots(fea):
+    clock = FakeClock()
+    dev = FakeDevice(clock, per_req=0.01)
+    reqs = _reqs(fea, 40, rate=400)
+    _sched(fea, clock, dev, slots=2).run(reqs)
+    # some batch was launched while the other slot was still busy
+    starts = [(s, t) for s, _, t in dev.launched]
+    assert {s for s, _ in starts} == {0, 1}
+
+
+def test_memory_budget_limits_batches(fea):
+    clock = FakeClock()
+    dev = FakeDevice(clock)
+    reqs = _reqs(fea, 30)
+    sch = _sched(fea, clock, dev, slots=2, max_batch=64, free=350, bytes_of=lambda r: 100)
+    done = sch.run(reqs)
+    assert len(done) == 30
+    assert sch.peak_in_flight <= 350
+    assert max(sch.batch_sizes) <= 3
+
+
+def test_request_larger_than_budget_is_rejected(fea):
+    clock = FakeClock()
+    dev = FakeDevice(clock)
+    with pytest.raises(MemoryError):
+        _sched(fea, clock, dev, free=50, bytes_of=lambda r: 100).run(_reqs(fea, 2))
+
+
+def test_plain_path_is_one_request_per_launch(fea):
+    clock = FakeClock()
+    dev = FakeDevice(clock)
+    done = _sched(fea, clock, dev, slots=1, max_batch=1).run(_reqs(fea, 10, rate=1000))
+    assert len(done) == 10 and all(len(b) == 1 for _, b, _ in dev.launched)
