@@ -90,8 +90,10 @@ class Renderer:
     ``save_for_backward=True`` keeps what ``backward`` needs in it.
     """
 
-    def __init__(self, tile: int = 16, batch: int = 64, mode: str = "culled", device="cuda"):
+    def __init__(self, tile: int = 16, batch: int = 64, mode: str = "culled", device="cuda",
+                 alpha_eps: float = 0.0, depth_eps: float = 0.0):
         self.lib = _lib.load()
+        self.alpha_eps, self.depth_eps = float(alpha_eps), float(depth_eps)
         self.device = torch.device(device)
         if self.device.type == "cuda" and self.device.index is None:
             self.device = torch.device("cuda", torch.cuda.current_device())
@@ -102,7 +104,7 @@ class Renderer:
 
     def _opt(self, save: bool, reuse: bool = False) -> FeOptions:
         return FeOptions(self.tile, self.batch, self.mode, int(save),
-                         _lib.FE_FLAG_REUSE_DESCRIPTORS if reuse else 0)
+                         _lib.FE_FLAG_REUSE_DESCRIPTORS if reuse else 0, self.alpha_eps, self.depth_eps)
 
     # -- planning -----------------------------------------------------------------------
     def _structs(self, scenes: Sequence[SceneTensors], views, outs):

@@ -52,7 +52,8 @@ class FeViewCotangent(ctypes.Structure):
 
 class FeOptions(ctypes.Structure):
     _fields_ = [("tile", ctypes.c_int32), ("batch", ctypes.c_int32), ("mode", ctypes.c_int32),
-                ("save_for_backward", ctypes.c_int32), ("flags", ctypes.c_int32)]
+                ("save_for_backward", ctypes.c_int32), ("flags", ctypes.c_int32),
+                ("alpha_eps", ctypes.c_float), ("depth_eps", ctypes.c_float)]
 
 
 FE_FLAG_REUSE_DESCRIPTORS = 1

@@ -23,6 +23,57 @@
 
 namespace fe {
 
+// Loop over the tile's pixels for one Gaussian (this lane), accumulating its 13 sums.
+// With v = t u - q, the t-derivative direction q - 2 t u equals -(v + t u), so
+//   g_u = -(g_t/vsv + g_d2 t) v - (g_t/vsv) t u.
+// SAFE zeroes non-finite per-pair terms (views with degenerate records only).
+template <bool SAFE>
+__device__ __forceinline__ void bwd_pixels(const float4* pxa, const float2* pxb, int npx, float a0, float a1,
+                                           float a2, float bb0, float bb1, float bb2, float g0, float g1, float g2,
+                                           float q0, float q1, float q2, float wl, float b3, float b2, float nb2l,
+                                           float kd, float& Gq0, float& Gq1, float& Gq2, float& Ga0, float& Ga1,
+                                           float& Ga2, float& Gb0, float& Gb1, float& Gb2, float& Gg0, float& Gg1,
+                                           float& Gg2, float& Gw) {
+  for (int p = 0; p < npx; ++p) {
+    const float4 pa = pxa[p];
+    const float2 pb = pxb[p];
+    const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y;
+    const float u0 = fmaf(a0, xf, fmaf(bb0, yf, g0));
+    const float u1 = fmaf(a1, xf, fmaf(bb1, yf, g1));
+    const float u2 = fmaf(a2, xf, fmaf(bb2, yf, g2));
+    const float vsv = fmaf(u0, u0, fmaf(u1, u1, u2 * u2));
+    const float psv = fmaf(q0, u0, fmaf(q1, u1, q2 * u2));
+    const float iv = __builtin_amdgcn_rcpf(vsv);
+    const float t = psv * iv;
+    const float v0 = fmaf(t, u0, -q0);
+    const float v1 = fmaf(t, u1, -q1);
+    const float v2 = fmaf(t, u2, -q2);
+    const float d0 = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
+    const float d2l = fmaf(kd, d0, wl);
+    const float e = __builtin_amdgcn_exp2f(d2l);
+    const float l = fmaf(b3, d2l, nb2l * t);
+    float s = __builtin_amdgcn_exp2f(l - M);
+    s = (t > 0.0f) ? s : 0.0f;
+    const float cs = cD * s;
+    const float dt = t - D;
+    float gd2 = fmaf(b3 * cs, dt, cE * e);
+    float gtv = cs * fmaf(-b2, dt, 1.0f) * iv;
+    if (SAFE && (!(gd2 == gd2) || !(gtv == gtv))) { gd2 = 0.0f; gtv = 0.0f; }
+    Gq0 = fmaf(gd2, v0, fmaf(gtv, u0, Gq0));
+    Gq1 = fmaf(gd2, v1, fmaf(gtv, u1, Gq1));
+    Gq2 = fmaf(gd2, v2, fmaf(gtv, u2, Gq2));
+    const float c1 = fmaf(gd2, t, gtv);     // g_t/vsv + g_d2 t
+    const float c2 = gtv * t;
+    const float gu0 = -fmaf(c1, v0, c2 * u0);
+    const float gu1 = -fmaf(c1, v1, c2 * u1);
+    const float gu2 = -fmaf(c1, v2, c2 * u2);
+    Ga0 = fmaf(gu0, xf, Ga0); Ga1 = fmaf(gu1, xf, Ga1); Ga2 = fmaf(gu2, xf, Ga2);
+    Gb0 = fmaf(gu0, yf, Gb0); Gb1 = fmaf(gu1, yf, Gb1); Gb2 = fmaf(gu2, yf, Gb2);
+    Gg0 += gu0; Gg1 += gu1; Gg2 += gu2;
+    Gw += gd2;
+  }
+}
+
 struct BwdArgs {
   const Job* jobs;
   const BwdJob* bjobs;
@@ -31,8 +82,11 @@ struct BwdArgs {
   const unsigned long long* off1;
   const unsigned long long* off2;
   const uint32_t* list;
+  const unsigned long long* mask1;
+  const unsigned long long* mask2;
   const float* st;
   int64_t total_pix;
+  const int32_t* job_flags;
   int dense;
 };
 
@@ -40,8 +94,11 @@ struct BwdArgs {
 // phase 2, or all Gaussians in dense mode); waves take chunks of 64 Gaussians round-robin.
 // Each lane loops over the tile's pixels, whose per-pixel terms (x', y', M, D, cE, cD) are
 // staged once in LDS and read as wave-uniform broadcasts, and keeps its Gaussian's 13 sums
-// in registers -- no cross-lane reduction.  The sums go to the per-(Gaussian, view)
-// accumulator with one atomic add each.
+// in registers -- no cross-lane reduction.  The chain rule then maps the sums to the 10
+// gradient components of (mu, upper U, w), added to the scene gradients with one
+// device-scope atomic each.  (Grouping several tiles per block to share those atomics was
+// measured slower: lanes of one wave then cover different numbers of tiles, or waves wait at
+// barriers.)
 template <int TS>
 __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   constexpr int NPIX = TS * TS;
@@ -67,7 +124,6 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   if (total == 0) return;
   const float ncnt = (float)total;
 
-  // per-pixel terms, packed over the valid pixels of the tile
   for (int i = threadIdx.x; i < npx; i += 256) {
     const int px = tx * TS + (i % wv), py = ty * TS + (i / wv);
     const int64_t lp = (int64_t)py * jb.W + px;
@@ -92,7 +148,9 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   const float nb2l = -jb.beta2 * kLog2e;
   const float kd = -0.5f * kLog2e;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const bool safe = (A.job_flags[ji] & 1) != 0;   // the view has degenerate records
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
+  const float ifx = 1.0f / jb.fx, ify = 1.0f / jb.fy;
 
   for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 256) {
     const int64_t kk = c0 + lane;
@@ -104,47 +162,14 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     const float4 r1 = rec4[gid * 4 + 1];
     const float4 r2 = rec4[gid * 4 + 2];
     const float4 r3 = rec4[gid * 4 + 3];
-    const float a0 = r0.x, a1 = r0.y, a2 = r0.z, bb0 = r0.w, bb1 = r1.x, bb2 = r1.y;
-    const float g0 = r1.z, g1 = r1.w, g2 = r2.x, q0 = r2.y, q1 = r2.z, q2 = r2.w, wl = r3.y;
     float Gq0 = 0.f, Gq1 = 0.f, Gq2 = 0.f, Ga0 = 0.f, Ga1 = 0.f, Ga2 = 0.f, Gb0 = 0.f, Gb1 = 0.f, Gb2 = 0.f;
     float Gg0 = 0.f, Gg1 = 0.f, Gg2 = 0.f, Gw = 0.f;
-    for (int p = 0; p < npx; ++p) {
-      const float4 pa = pxa[p];
-      const float2 pb = pxb[p];
-      const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y;
-      const float u0 = fmaf(a0, xf, fmaf(bb0, yf, g0));
-      const float u1 = fmaf(a1, xf, fmaf(bb1, yf, g1));
-      const float u2 = fmaf(a2, xf, fmaf(bb2, yf, g2));
-      const float vsv = fmaf(u0, u0, fmaf(u1, u1, u2 * u2));
-      const float psv = fmaf(q0, u0, fmaf(q1, u1, q2 * u2));
-      const float iv = __builtin_amdgcn_rcpf(vsv);
-      const float t = psv * iv;
-      const float v0 = fmaf(t, u0, -q0);
-      const float v1 = fmaf(t, u1, -q1);
-      const float v2 = fmaf(t, u2, -q2);
-      const float d0 = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
-      const float d2l = fmaf(kd, d0, wl);
-      const float e = __builtin_amdgcn_exp2f(d2l);
-      const float l = fmaf(b3, d2l, nb2l * t);
-      float s = __builtin_amdgcn_exp2f(l - M);
-      s = (t > 0.0f) ? s : 0.0f;
-      const float cs = cD * s;
-      const float dt = t - D;
-      float gd2 = fmaf(b3 * cs, dt, cE * e);
-      float gtv = cs * fmaf(-b2, dt, 1.0f) * iv;
-      if (!(gd2 == gd2) || !(gtv == gtv)) { gd2 = 0.0f; gtv = 0.0f; }
-      Gq0 = fmaf(gd2, v0, fmaf(gtv, u0, Gq0));
-      Gq1 = fmaf(gd2, v1, fmaf(gtv, u1, Gq1));
-      Gq2 = fmaf(gd2, v2, fmaf(gtv, u2, Gq2));
-      const float m2t = -2.0f * t;
-      const float gdt = gd2 * t;
-      const float gu0 = fmaf(gtv, fmaf(m2t, u0, q0), -(gdt * v0));
-      const float gu1 = fmaf(gtv, fmaf(m2t, u1, q1), -(gdt * v1));
-      const float gu2 = fmaf(gtv, fmaf(m2t, u2, q2), -(gdt * v2));
-      Ga0 = fmaf(gu0, xf, Ga0); Ga1 = fmaf(gu1, xf, Ga1); Ga2 = fmaf(gu2, xf, Ga2);
-      Gb0 = fmaf(gu0, yf, Gb0); Gb1 = fmaf(gu1, yf, Gb1); Gb2 = fmaf(gu2, yf, Gb2);
-      Gg0 += gu0; Gg1 += gu1; Gg2 += gu2;
-      Gw += gd2;
+    if (safe) {
+      bwd_pixels<true>(pxa, pxb, npx, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w,
+                       r3.y, b3, b2, nb2l, kd, Gq0, Gq1, Gq2, Ga0, Ga1, Ga2, Gb0, Gb1, Gb2, Gg0, Gg1, Gg2, Gw);
+    } else {
+      bwd_pixels<false>(pxa, pxb, npx, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w,
+                        r3.y, b3, b2, nb2l, kd, Gq0, Gq1, Gq2, Ga0, Ga1, Ga2, Gb0, Gb1, Gb2, Gg0, Gg1, Gg2, Gw);
     }
     // chain rule for this (tile, Gaussian) pair, straight into the scene gradients:
     //   dM = [G_a/fx, G_b/fy, G_g],  dA = dM R + G_q p^T (lower part),  dmu = A^T G_q,
@@ -152,11 +177,9 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     const float* P = jb.prec + 9 * gid;
     const float* mu = jb.means + 3 * gid;
     const float u00 = P[0], u01 = P[1], u02 = P[2], u11 = P[4], u12 = P[5], u22 = P[8];
-    const float p0 = mu[0] - jb.o[0], p1 = mu[1] - jb.o[1], p2 = mu[2] - jb.o[2];
-    const float ifx = 1.0f / jb.fx, ify = 1.0f / jb.fy;
+    const float pv[3] = {mu[0] - jb.o[0], mu[1] - jb.o[1], mu[2] - jb.o[2]};
     const float dm[3][3] = {{Ga0 * ifx, Gb0 * ify, Gg0}, {Ga1 * ifx, Gb1 * ify, Gg1}, {Ga2 * ifx, Gb2 * ify, Gg2}};
     const float gq[3] = {Gq0, Gq1, Gq2};
-    const float pv[3] = {p0, p1, p2};
     float dA[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -195,12 +218,12 @@ static hipError_t bwd_t(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
   return hipGetLastError();
 }
 
-hipError_t launch_bwd_blend(hipStream_t s, int ts, int gb, int64_t total_tiles, const Job* jobs,
-                            const void* bjobs, int n_jobs, const float* rec, const unsigned long long* off1,
+hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
+                            int n_jobs, const float* rec, const unsigned long long* off1,
                             const unsigned long long* off2, const uint32_t* list, const float* st,
-                            int64_t total_pix, int dense) {
-  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, off1, off2, list, st, total_pix, dense};
-  (void)gb;
+                            int64_t total_pix, const int32_t* job_flags, int dense) {
+  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, off1, off2, list, nullptr, nullptr, st,
+            total_pix, job_flags, dense};
   switch (ts) {
     case 8: return bwd_t<8>(s, total_tiles, a);
     case 16: return bwd_t<16>(s, total_tiles, a);

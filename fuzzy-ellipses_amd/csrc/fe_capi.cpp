@@ -63,9 +63,15 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     set_msg("fe_plan: need at least one scene and one view");
     return FE_ERR_INVALID_ARGUMENT;
   }
-  fe_options o = {16, 64, FE_MODE_CULLED, 0, 0};
+  fe_options o = {16, 64, FE_MODE_CULLED, 0, 0, 0.0f, 0.0f};
   if (opt) o = *opt;
   if (o.tile == 0) o.tile = 16;
+  if (!(o.alpha_eps > 0.0f)) o.alpha_eps = 2e-5f;
+  if (!(o.depth_eps > 0.0f)) o.depth_eps = 1e-6f;
+  if (!(o.alpha_eps < 1.0f) || !(o.depth_eps < 1.0f)) {
+    set_msg("fe_plan: alpha_eps and depth_eps must be in (0, 1)");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
   if (o.batch == 0) o.batch = 64;
   if (o.tile != 8 && o.tile != 16 && o.tile != 32) {
     set_msg("fe_plan: tile must be 8, 16 or 32");
@@ -122,10 +128,10 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       jb.fx = c.fx; jb.fy = c.fy; jb.cx = c.cx; jb.cy = c.cy;
       jb.beta2 = vw.beta2; jb.beta3 = vw.beta3;
       const double ln_n = log((double)std::max(sc.n, 1));
-      // alpha: total dropped density <= n * exp(-c_alpha) = exp(-14) ~ 8e-7
-      jb.c_alpha = (float)(ln_n + 14.0);
-      // depth: dropped softmax weight fraction <= n * exp(-delta) = exp(-20) ~ 2e-9
-      jb.delta2 = (float)((ln_n + 20.0) * 1.4426950408889634);
+      // alpha: total dropped density <= n * exp(-c_alpha) = alpha_eps
+      jb.c_alpha = (float)(ln_n - log((double)o.alpha_eps));
+      // depth: dropped softmax weight fraction <= n * exp(-delta) = depth_eps
+      jb.delta2 = (float)((ln_n - log((double)o.depth_eps)) * 1.4426950408889634);
       jb.floor_thr = (float)(13.0 * 3.321928094887362 - ln_n * 1.4426950408889634);
       jb.n = sc.n; jb.H = c.height; jb.W = c.width;
       jb.tiles_x = tx; jb.tiles_y = ty; jb.n_tiles = tx * ty;
@@ -317,7 +323,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
       return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, nullptr, nullptr, nullptr, state,
-                          L.total_pix, lmin, ubm, cnt, L.save);
+                          L.total_pix, lmin, ubm, cnt, flags, L.save);
     }));
     return FE_OK;
   }
@@ -337,7 +343,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   FE_HIP(timed(s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }));
   FE_HIP(timed(s, FE_K_BLEND1, [&] {
     return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state, L.total_pix,
-                        lmin, ubm, cnt, L.save);
+                        lmin, ubm, cnt, flags, L.save);
   }));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
@@ -350,7 +356,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   FE_HIP(timed(s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }));
   FE_HIP(timed(s, FE_K_BLEND2, [&] {
     return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state, L.total_pix,
-                        lmin, ubm, cnt, L.save);
+                        lmin, ubm, cnt, flags, L.save);
   }));
   return FE_OK;
 }
@@ -358,7 +364,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
 fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
                    int32_t n_views, const fe_options* opt, const fe_view_cotangent* cot,
                    const fe_scene_grad* grads, void* ws, size_t ws_bytes) {
-  fe_options o = {16, 64, FE_MODE_CULLED, 1, 0};
+  fe_options o = {16, 64, FE_MODE_CULLED, 1, 0, 0.0f, 0.0f};
   if (opt) o = *opt;
   o.save_for_backward = 1;
   Layout L;
@@ -390,11 +396,11 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
   const void* dbj = at<char>(ws, L.o_bjobs);
   const bool dense = (L.mode == FE_MODE_DENSE);
   FE_HIP(timed(s, FE_K_BWD_BLEND, [&] {
-    return launch_bwd_blend(s, L.ts, L.gb, L.total_tiles, dj, dbj, L.n_jobs, at<float>(ws, L.o_rec),
+    return launch_bwd_blend(s, L.ts, L.total_tiles, dj, dbj, L.n_jobs, at<float>(ws, L.o_rec),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off1),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off2),
                             dense ? nullptr : at<uint32_t>(ws, L.o_list), at<float>(ws, L.o_state), L.total_pix,
-                            dense ? 1 : 0);
+                            at<int32_t>(ws, L.o_flags), dense ? 1 : 0);
   }));
   return FE_OK;
 }
@@ -505,7 +511,7 @@ fe_status fe_render_workspace_bytes(int32_t n, int32_t height, int32_t width, in
   v.cam.height = height;
   v.cam.width = width;
   v.cam.fx = v.cam.fy = 1.0f;
-  fe_options o{tile, 64, FE_MODE_CULLED, 0, 0};
+  fe_options o{tile, 64, FE_MODE_CULLED, 0, 0, 0.0f, 0.0f};
   return fe_plan(&sc, 1, &v, 1, &o, bytes);
 }
 
@@ -521,7 +527,7 @@ fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float
   v.beta3 = beta3;
   v.depth = depth;
   v.alpha = alpha;
-  fe_options o{16, 64, FE_MODE_CULLED, 0, 0};
+  fe_options o{16, 64, FE_MODE_CULLED, 0, 0, 0.0f, 0.0f};
   return fe_forward(stream, &sc, 1, &v, 1, &o, workspace, workspace_bytes);
 }
 

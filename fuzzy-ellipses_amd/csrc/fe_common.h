@@ -115,11 +115,12 @@ __device__ __forceinline__ TileGeo tile_geo_xy(const Job& jb, int tx, int ty, in
 //   t_lb         : lower bound of t* where the whole tile lies on the q side (else 0)
 //   plus         : true when every ray of the tile has q.u > 0 (t* > 0 on the tile)
 //   t_any_lb     : lower bound of t* on the tile regardless of side (-inf if unknown)
+//   t_ub         : upper bound of t* on the tile (+inf if unknown)
 //   unbounded    : the tile cone is too wide for the bound (keep the pair)
 // sqrt(d0) = |q| sin(angle(q, line)) is |q|-Lipschitz in the ray angle; the angle from the
 // centre ray to any tile ray is at most the largest corner angle, bounded by its tangent.
 struct PairBound {
-  float sd_lo, sd_hi, t_lb, t_any_lb;
+  float sd_lo, sd_hi, t_lb, t_any_lb, t_ub;
   bool plus, unbounded;
 };
 
@@ -167,6 +168,8 @@ __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& t
   pb.t_lb = pb.plus ? sqrtf(fmaxf(fmaf(-pb.sd_hi, pb.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
   // |q.u|/|u| <= sqrt(qq - d0_lo) on the tile, |u| >= |u_c| - rho
   pb.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-pb.sd_lo, pb.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
+  // upper bound of t* on the tile: q.u/|u| <= min(|q|, q.u_c/|u_c| + |q| tan(angle)), |u| >= |u_c| - rho
+  pb.t_ub = (un > rho) ? fminf(qn, qu / un + dth) / (un - rho) : INFINITY;
   if (!(pb.sd_lo == pb.sd_lo)) pb.unbounded = true;  // NaN anywhere -> keep
   return pb;
 }

@@ -169,6 +169,44 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
     }
   }
   __syncthreads();
+  const bool dense_job = (job_flags[ji] & 1) != 0;
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
+  if (PHASE == 2 && !dense_job) {
+    // A sub-tile with a pixel that no phase-1 pair covers on the visible side has
+    // lmin = -inf.  Its threshold then comes from bounds: any Gaussian entirely in front of
+    // the tile's rays has logit >= lb = b3 (w - sd_hi^2/2) - b2 t_ub at every pixel, so the
+    // log2-sum-exp is at least max_j lb_j there (and that Gaussian passes the test below).
+    __shared__ float sh_lb[4];
+    for (int k = 0; k < nsub; ++k) {
+      if (!(lmin[k] == -INFINITY)) continue;   // block-uniform
+      float lb = -INFINITY;
+      for (int c = wave; c < jb.nw; c += 4) {
+        const int j = c * 64 + lane;
+        if (j >= jb.n) continue;
+        const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
+        float r[16];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float4 t4 = r4[i];
+          r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
+        }
+        if (__float_as_int(r[15]) & 1) continue;
+        const PairBound pb = pair_bound(r, tg[k]);
+        if (pb.plus && !pb.unbounded)
+          lb = fmaxf(lb, fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pb.t_ub)));
+      }
+      for (int o = 32; o > 0; o >>= 1) lb = fmaxf(lb, __shfl_xor(lb, o, 64));
+      if (lane == 0) sh_lb[wave] = lb;
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(sh_lb[0], sh_lb[1]), fmaxf(sh_lb[2], sh_lb[3]));
+        lmin[k] = m;
+        tau[k] = m - jb.delta2;
+      }
+      __syncthreads();
+    }
+  }
   if (PHASE == 2 && threadIdx.x == 0) {
     float m = INFINITY;
     for (int k = 0; k < nsub; ++k) m = fminf(m, tau[k]);
@@ -176,9 +214,6 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
   }
   __syncthreads();
   const float tau_s = (PHASE == 2) ? tau_grp : 0.0f;
-  const bool dense_job = (job_flags[ji] & 1) != 0;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
   uint32_t cnt[NS];
   float ubm[NS];
 #pragma unroll
@@ -401,6 +436,63 @@ __global__ __launch_bounds__(256) void k_tile_write(const Job* __restrict__ jobs
 // Per pixel state: M (running max of log2 logits), S = sum mask*2^(l-M), Z = sum
 // mask*2^(l-M)*t, E = sum exp(d2), Zall = sum nan_to_num(t), nan flag.
 // ------------------------------------------------------------------------------------------
+// One LDS batch of `nbr` records (a multiple of SUB) for this thread's pixel.  The online
+// maximum is updated once per SUB records.  SAFE keeps the exact NaN/inf semantics
+// (nan_to_num of t for the weight floor, NaN-logit flag); it is only needed for views with
+// degenerate records, whose t* or logits can be non-finite.
+template <int SUB, bool SAFE>
+__device__ __forceinline__ void blend_batch(const float4* lds, int nbr, float xf, float yf, float b3, float nb2l,
+                                            float kd, float& M, float& S, float& Z, float& E, float& Zall,
+                                            bool& nanf) {
+  for (int k0 = 0; k0 < nbr; k0 += SUB) {
+    float ls[SUB], tsv[SUB];
+    float mx = M;
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      const float4 r0 = lds[(k0 + s) * 4 + 0];
+      const float4 r1 = lds[(k0 + s) * 4 + 1];
+      const float4 r2 = lds[(k0 + s) * 4 + 2];
+      const float4 r3 = lds[(k0 + s) * 4 + 3];
+      // a = r0.xyz, b = (r0.w, r1.x, r1.y), g = (r1.z, r1.w, r2.x), q = (r2.y, r2.z, r2.w)
+      const float u0 = fmaf(r0.x, xf, fmaf(r0.w, yf, r1.z));
+      const float u1 = fmaf(r0.y, xf, fmaf(r1.x, yf, r1.w));
+      const float u2 = fmaf(r0.z, xf, fmaf(r1.y, yf, r2.x));
+      const float vsv = fmaf(u0, u0, fmaf(u1, u1, u2 * u2));
+      const float psv = fmaf(r2.y, u0, fmaf(r2.z, u1, r2.w * u2));
+      const float t = psv * __builtin_amdgcn_rcpf(vsv);
+      const float v0 = fmaf(t, u0, -r2.y);
+      const float v1 = fmaf(t, u1, -r2.z);
+      const float v2 = fmaf(t, u2, -r2.w);
+      const float d0 = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
+      const float d2l = fmaf(kd, d0, r3.y);           // log2(e) * d2
+      E += __builtin_amdgcn_exp2f(d2l);
+      const float l = fmaf(b3, d2l, nb2l * t);        // log2(e) * (b3 d2 - b2 t)
+      if (SAFE) {
+        nanf = nanf || (l != l);
+        Zall += nan_to_num(t);
+      } else {
+        Zall += t;
+      }
+      ls[s] = l;
+      tsv[s] = t;
+      mx = fmaxf(mx, l);
+    }
+    if (mx > M) {
+      const float sc = __builtin_amdgcn_exp2f(M - mx);
+      S *= sc;
+      Z *= sc;
+      M = mx;
+    }
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      float sv = __builtin_amdgcn_exp2f(ls[s] - M);
+      sv = (tsv[s] > 0.0f) ? sv : 0.0f;
+      S += sv;
+      Z = fmaf(sv, tsv[s], Z);
+    }
+  }
+}
+
 struct BlendArgs {
   const Job* jobs;
   int n_jobs;
@@ -413,6 +505,7 @@ struct BlendArgs {
   float* tile_lmin;
   const float* tile_ubm;
   Counters* cnt;
+  const int32_t* job_flags;
   int save;
 };
 
@@ -437,6 +530,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   const float b3 = jb.beta3;
   const float nb2l = -jb.beta2 * kLog2e;
   const float kd = -0.5f * kLog2e;
+  const bool safe = (A.job_flags[ji] & 1) != 0;   // the view has degenerate records
 
   float M, S, Z, E, Zall;
   if (PHASE == 2) {
@@ -474,49 +568,8 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
       lds[i] = v;
     }
     __syncthreads();
-    for (int k0 = 0; k0 < nbr; k0 += SUB) {
-      float ls[SUB], tsv[SUB];
-      float mx = M;
-#pragma unroll
-      for (int s = 0; s < SUB; ++s) {
-        const float4 r0 = lds[(k0 + s) * 4 + 0];
-        const float4 r1 = lds[(k0 + s) * 4 + 1];
-        const float4 r2 = lds[(k0 + s) * 4 + 2];
-        const float4 r3 = lds[(k0 + s) * 4 + 3];
-        // a = r0.xyz, b = (r0.w, r1.x, r1.y), g = (r1.z, r1.w, r2.x), q = (r2.y, r2.z, r2.w)
-        const float u0 = fmaf(r0.x, xf, fmaf(r0.w, yf, r1.z));
-        const float u1 = fmaf(r0.y, xf, fmaf(r1.x, yf, r1.w));
-        const float u2 = fmaf(r0.z, xf, fmaf(r1.y, yf, r2.x));
-        const float vsv = fmaf(u0, u0, fmaf(u1, u1, u2 * u2));
-        const float psv = fmaf(r2.y, u0, fmaf(r2.z, u1, r2.w * u2));
-        const float t = psv * __builtin_amdgcn_rcpf(vsv);
-        const float v0 = fmaf(t, u0, -r2.y);
-        const float v1 = fmaf(t, u1, -r2.z);
-        const float v2 = fmaf(t, u2, -r2.w);
-        const float d0 = fmaf(v0, v0, fmaf(v1, v1, v2 * v2));
-        const float d2l = fmaf(kd, d0, r3.y);           // log2(e) * d2
-        E += __builtin_amdgcn_exp2f(d2l);
-        const float l = fmaf(b3, d2l, nb2l * t);        // log2(e) * (b3 d2 - b2 t)
-        nanf = nanf || (l != l);
-        Zall += nan_to_num(t);
-        ls[s] = l;
-        tsv[s] = t;
-        mx = fmaxf(mx, l);
-      }
-      if (mx > M) {
-        const float sc = __builtin_amdgcn_exp2f(M - mx);
-        S *= sc;
-        Z *= sc;
-        M = mx;
-      }
-#pragma unroll
-      for (int s = 0; s < SUB; ++s) {
-        float sv = __builtin_amdgcn_exp2f(ls[s] - M);
-        sv = (tsv[s] > 0.0f) ? sv : 0.0f;
-        S += sv;
-        Z = fmaf(sv, tsv[s], Z);
-      }
-    }
+    if (safe) blend_batch<SUB, true>(lds, nbr, xf, yf, b3, nb2l, kd, M, S, Z, E, Zall, nanf);
+    else blend_batch<SUB, false>(lds, nbr, xf, yf, b3, nb2l, kd, M, S, Z, E, Zall, nanf);
     __syncthreads();
   }
 
@@ -669,8 +722,9 @@ static hipError_t launch_blend_t(hipStream_t s, int gb, int phase, int64_t total
 hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_tiles, const Job* jobs,
                         int n_jobs, const float* rec, const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
-                        float* tile_lmin, const float* tile_ubm, Counters* cnt, int save) {
-  BlendArgs a{jobs, n_jobs, rec, off1, off2, list, st, total_pix, tile_lmin, tile_ubm, cnt, save};
+                        float* tile_lmin, const float* tile_ubm, Counters* cnt, const int32_t* job_flags,
+                        int save) {
+  BlendArgs a{jobs, n_jobs, rec, off1, off2, list, st, total_pix, tile_lmin, tile_ubm, cnt, job_flags, save};
   switch (ts) {
     case 8: return launch_blend_t<8>(s, gb, phase, total_tiles, a);
     case 16: return launch_blend_t<16>(s, gb, phase, total_tiles, a);

@@ -26,11 +26,12 @@ hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t
 hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_tiles, const Job* jobs,
                         int n_jobs, const float* rec, const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
-                        float* tile_lmin, const float* tile_ubm, Counters* cnt, int save);
+                        float* tile_lmin, const float* tile_ubm, Counters* cnt, const int32_t* job_flags,
+                        int save);
 
-hipError_t launch_bwd_blend(hipStream_t s, int ts, int gb, int64_t total_tiles, const Job* jobs,
-                            const void* bjobs, int n_jobs, const float* rec, const unsigned long long* off1,
+hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
+                            int n_jobs, const float* rec, const unsigned long long* off1,
                             const unsigned long long* off2, const uint32_t* list, const float* st,
-                            int64_t total_pix, int dense);
+                            int64_t total_pix, const int32_t* job_flags, int dense);
 
 }  // namespace fe

@@ -106,6 +106,11 @@ typedef struct fe_options {
   int32_t mode;        /* FE_MODE_* */
   int32_t save_for_backward; /* keep per-pixel aggregates for fe_backward */
   int32_t flags;       /* FE_FLAG_* */
+  /* Culling error budgets (0 = default).  Pairs are dropped only where the dropped alpha
+   * mass is provably at most alpha_eps (absolute, per pixel; default 2e-5) and the dropped
+   * share of the depth softmax weight at most depth_eps (default 1e-6). */
+  float alpha_eps;
+  float depth_eps;
 } fe_options;
 
 /* Diagnostics written by fe_forward into the workspace and readable with fe_read_stats. */

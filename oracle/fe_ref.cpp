@@ -222,7 +222,7 @@ void fe_ref_records(int n, const float* means, const float* prec9, const float* 
 }
 
 struct Bound {
-  float sd_lo, sd_hi, t_lb, t_any_lb;
+  float sd_lo, sd_hi, t_lb, t_any_lb, t_ub;
   bool plus, unbounded;
 };
 
@@ -266,6 +266,7 @@ static Bound bound_of(const float* r, float Xc, float Yc, float hx, float hy) {
   o.plus = (qu > 0.0f) && (o.sd_hi < qn);
   o.t_lb = o.plus ? sqrtf(fmaxf(fmaf(-o.sd_hi, o.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
   o.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-o.sd_lo, o.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
+  o.t_ub = (un > rho) ? fminf(qn, qu / un + dth) / (un - rho) : INFINITY;
   if (!(o.sd_lo == o.sd_lo)) o.unbounded = true;
   return o;
 }
@@ -298,6 +299,29 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
   const float b3l = beta3 * kLog2e, b2l = beta2 * kLog2e;
   int64_t pos = 0;
   std::vector<char> in1(n > 0 ? n : 1, 0);
+  // phase-1 thresholds, with the bound-based replacement for tiles whose value is -inf
+  std::vector<float> lm;
+  if (tile_lmin) {
+    lm.assign(tile_lmin, tile_lmin + tx * ty);
+    if (!any_degenerate) {
+      for (int tile = 0; tile < tx * ty; ++tile) {
+        if (!(lm[tile] == -INFINITY)) continue;
+        const Geo g = geo_of(tile % tx, tile / tx, ts, W, H, cx, cy);
+        float lb = -INFINITY;
+        for (int j = 0; j < n; ++j) {
+          const float* r = rec + 16 * (size_t)j;
+          int32_t fl;
+          memcpy(&fl, &r[15], 4);
+          if (fl & 1) continue;
+          const Bound o = bound_of(r, g.Xc, g.Yc, g.hx, g.hy);
+          if (o.plus && !o.unbounded)
+            lb = fmaxf(lb, fmaf(b3l, fmaf(-0.5f * o.sd_hi, o.sd_hi, r[12]), -(b2l * o.t_ub)));
+        }
+        lm[tile] = lb;
+      }
+    }
+    tile_lmin = lm.data();
+  }
   for (int tile = 0; tile < tx * ty; ++tile) {
     const int txi = tile % tx, tyi = tile / tx;
     const Geo g = geo_of(txi, tyi, ts, W, H, cx, cy);

@@ -107,10 +107,13 @@ def records(scene, cam):
     return rec[:n], int(deg.value)
 
 
-def cull_constants(n):
-    """(c_alpha, delta2, floor_thr) exactly as the planner computes them (float32 of float64)."""
+def cull_constants(n, alpha_eps=2e-5, depth_eps=1e-6):
+    """(c_alpha, delta2, floor_thr) exactly as the planner computes them (float32 of float64
+    math, with the budgets rounded to float32 as the C ABI stores them)."""
     ln_n = np.log(max(n, 1))
-    return (float(np.float32(ln_n + 14.0)), float(np.float32((ln_n + 20.0) * LOG2E)),
+    ae = float(np.float32(alpha_eps))
+    de = float(np.float32(depth_eps))
+    return (float(np.float32(ln_n - np.log(ae))), float(np.float32((ln_n - np.log(de)) * LOG2E)),
             float(np.float32(13.0 * 3.321928094887362 - ln_n * LOG2E)))
 
 

@@ -88,3 +88,40 @@ def test_python_api_fails_loudly_without_library(fea, monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     with pytest.raises(_lib.FeError):
         _lib.load(str(tmp_path / "missing.so"))
+
+
+def test_error_budgets_validated_and_monotone(fea):
+    from fuzzy_ellipses_amd import _lib
+    lib = _lib.load()
+    sc = (_lib.FeScene * 1)(_lib.FeScene(100, None, None, None))
+    cam = _lib.FeCamera()
+    cam.fx = cam.fy = 64.0
+    cam.height = cam.width = 64
+    vw = (_lib.FeView * 1)(_lib.FeView(0, cam, 21.4, 2.66, None, None))
+    nbytes = ctypes.c_size_t(0)
+    bad = _lib.FeOptions(16, 64, 0, 0, 0, 1.5, 0.0)
+    assert lib.fe_plan(sc, 1, vw, 1, ctypes.byref(bad), ctypes.byref(nbytes)) == 1
+    ok = _lib.FeOptions(16, 64, 0, 0, 0, 1e-3, 1e-4)
+    assert lib.fe_plan(sc, 1, vw, 1, ctypes.byref(ok), ctypes.byref(nbytes)) == 0
+
+
+def test_looser_alpha_budget_lists_fewer_pairs(fea, scenes_mod):
+    """The CPU restatement of the membership with the planner's constants: a looser budget
+    never lists more pairs (the keep region shrinks monotonically)."""
+    import numpy as np
+    from oracle import ref
+    scene = scenes_mod.single_view_scene(seed=4, n=80, H=64, W=64, f=64.0)
+    cam = scene.cameras[0]
+    rec, deg = ref.records(scene, cam)
+    lib = ref.load()
+    counts = []
+    for eps in (1e-7, 2e-5, 1e-3):
+        c_alpha, _, _ = ref.cull_constants(scene.n, alpha_eps=eps)
+        T = 16
+        c = np.zeros(T, np.uint32)
+        lst = np.zeros(scene.n * T, np.uint32)
+        tot = lib.fe_ref_tiles(scene.n, np.ascontiguousarray(rec.reshape(-1)), deg, 64, 64, 16, np.float32(cam.cx),
+                               np.float32(cam.cy), np.float32(scene.beta2), np.float32(scene.beta3),
+                               np.float32(c_alpha), np.float32(1.0), np.float32(40.0), None, c, lst)
+        counts.append(tot)
+    assert counts[0] >= counts[1] >= counts[2]

@@ -65,8 +65,17 @@ def test_phase1_and_phase2_are_conservative(scenes_mod, seed, ts):
     m2, _ = _member(counts2, lst2, T, scene.n)
     assert not np.any(m1 & m2), "phase 2 repeats a phase-1 pair"
     union = (m1 | m2)[tile]
-    need_depth = (zs > 0) & (l2 >= lse[:, None] - delta2 + 0.5)
+    # the guarantee: every dropped unmasked pair's logit is below the log2-sum-exp of the
+    # final listed set minus the margin (so dropped weight <= n * 2^-delta2 of the listed)
+    lu = np.where(union & (zs > 0), l2, -np.inf)
+    mu_ = lu.max(1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        lse_u = mu_ + np.log2(np.sum(np.exp2(lu - mu_[:, None]), 1))
+    lse_u[~np.isfinite(mu_)] = -np.inf
+    need_depth = (zs > 0) & (l2 >= lse_u[:, None] - delta2 + 0.5)
     assert np.all(union[need_depth]), "phase 1+2 miss a pair the depth bound needs"
+    # pixels whose phase-1 coverage was empty still end up with a finite listed mass
+    assert np.all(np.isfinite(lse_u[~np.isfinite(lse)])), "uncovered pixels stay uncovered"
     # and culling actually removes work on these scenes
     assert union.mean() < 1.0
 
