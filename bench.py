@@ -28,6 +28,21 @@ FLOPS_FWD = 36                    # per evaluated (pixel, Gaussian) pair, blend 
 FLOPS_BWD = 93                    # per evaluated pair, blend backward incl. recompute
 
 
+
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "traffic_shell_n4096_16x480x640.json")
+
+
+def measured_traffic(dom, args):
+    """HBM bytes per launch of kernel `dom` (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a copy kernel)
+    from the committed rocprofv3 --pmc summary of this workload; None for a configuration it does not cover."""
+    if args.tile != 16 or args.batch != 64 or args.mode != "culled" or not os.path.exists(TRAFFIC_FILE):
+        return None
+    names = {"bwd_blend": "fe::k_bwd_blend<16>", "blend1": "fe::k_blend<16, 64, 1>", "blend2": "fe::k_blend<16, 64, 2>"}
+    with open(TRAFFIC_FILE) as f:
+        k = json.load(f)["kernels"].get(names.get(dom, ""))
+    return None if k is None else int(k["read_bytes"] + k["write_bytes"])
+
 def _package():
     if "fuzzy_ellipses_amd" in sys.modules:
         return sys.modules["fuzzy_ellipses_amd"]
@@ -307,7 +322,7 @@ def main():
     avg_s = kavg[dom] / 1e3
     achieved = flops_per_launch[dom] / avg_s / 1e12 if avg_s > 0 else 0.0
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None, "kernel": dom,
+            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, args), "kernel": dom,
             "note": "compute-bound on FP32 VALU (no MFMA); frac of unpacked FP32 peak 78.6 = "
                     f"{achieved / FP32_PEAK_UNPACKED_TFLOPS:.4f}; flops/launch = "
                     f"{FLOPS_FWD if dom != 'bwd_blend' else FLOPS_BWD} x evaluated pairs"}
