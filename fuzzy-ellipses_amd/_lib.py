@@ -18,7 +18,7 @@ FE_MODE_DENSE = 1
 EXPORTS = [
     "fe_version", "fe_status_string", "fe_last_error_string", "fe_plan", "fe_forward",
     "fe_backward", "fe_read_stats", "fe_render_workspace_bytes", "fe_render_fwd", "fe_profile_enable",
-    "fe_profile_read",
+    "fe_profile_read", "fe_silhouette_workspace_bytes", "fe_silhouette_loss", "fe_adam_step",
 ]
 KERNELS = ["preprocess", "mask1", "scan1", "write1", "blend1", "mask2", "scan2", "write2", "blend2",
            "blend_dense", "bwd_blend", "bwd_finalize"]
@@ -57,6 +57,12 @@ class FeOptions(ctypes.Structure):
 
 
 FE_FLAG_REUSE_DESCRIPTORS = 1
+
+
+class FeSilView(ctypes.Structure):
+    _fields_ = [("alpha", ctypes.c_void_p), ("depth", ctypes.c_void_p), ("mask", ctypes.c_void_p),
+                ("target_depth", ctypes.c_void_p), ("d_alpha", ctypes.c_void_p), ("d_depth", ctypes.c_void_p),
+                ("n_pixels", ctypes.c_int64)]
 
 
 class FeStats(ctypes.Structure):
@@ -114,6 +120,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fe_profile_enable.argtypes = [ctypes.c_int32]
     lib.fe_profile_read.restype = ctypes.c_int
     lib.fe_profile_read.argtypes = [ctypes.POINTER(FeKernelTimes), ctypes.c_int32]
+    lib.fe_silhouette_workspace_bytes.restype = ctypes.c_int
+    lib.fe_silhouette_workspace_bytes.argtypes = [ctypes.c_int32, ctypes.POINTER(ctypes.c_size_t)]
+    lib.fe_silhouette_loss.restype = ctypes.c_int
+    lib.fe_silhouette_loss.argtypes = [ctypes.c_void_p, ctypes.POINTER(FeSilView), ctypes.c_int32, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p,
+                                       ctypes.c_size_t]
+    lib.fe_adam_step.restype = ctypes.c_int
+    lib.fe_adam_step.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                 ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_int32]
     _lib = lib
     return lib
 

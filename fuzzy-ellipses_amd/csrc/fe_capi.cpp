@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <cmath>
 #include <mutex>
 #include <vector>
 
@@ -529,6 +530,65 @@ fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float
   v.alpha = alpha;
   fe_options o{16, 64, FE_MODE_CULLED, 0, 0, 0.0f, 0.0f};
   return fe_forward(stream, &sc, 1, &v, 1, &o, workspace, workspace_bytes);
+}
+
+static size_t sil_align(size_t x) { return (x + 255) & ~(size_t)255; }
+
+fe_status fe_silhouette_workspace_bytes(int32_t n_views, size_t* bytes) {
+  if (!bytes || n_views <= 0 || n_views > (1 << 20)) return FE_ERR_INVALID_ARGUMENT;
+  const size_t part = sizeof(float) * (size_t)n_views * fe::kSilBlocksPerView;
+  *bytes = sil_align(sizeof(fe_sil_view) * (size_t)n_views) + 2 * sil_align(part);
+  return FE_OK;
+}
+
+fe_status fe_silhouette_loss(void* stream, const fe_sil_view* views, int32_t n_views, float w_depth, float clip,
+                             float* loss, int32_t flags, void* workspace, size_t workspace_bytes) {
+  size_t need = 0;
+  if (!views || !loss || fe_silhouette_workspace_bytes(n_views, &need) != FE_OK) {
+    fe::set_msg("fe_silhouette_loss: need views, a loss pointer and 1 <= n_views <= 2^20");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  if (!(clip >= 0.0f && clip < 0.5f) || !(w_depth >= 0.0f)) {
+    fe::set_msg("fe_silhouette_loss: clip must be in [0, 0.5) and w_depth >= 0");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  for (int32_t i = 0; i < n_views; ++i) {
+    const fe_sil_view& v = views[i];
+    if (!v.alpha || !v.depth || !v.mask || !v.target_depth || !v.d_alpha || !v.d_depth || v.n_pixels <= 0) {
+      fe::set_msg("fe_silhouette_loss: null buffer or empty view");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+  }
+  if (!workspace || workspace_bytes < need) {
+    fe::set_msg("fe_silhouette_loss: workspace missing or too small (see fe_silhouette_workspace_bytes)");
+    return FE_ERR_WORKSPACE;
+  }
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  char* base = reinterpret_cast<char*>(workspace);
+  fe_sil_view* d_views = reinterpret_cast<fe_sil_view*>(base);
+  const size_t part = sil_align(sizeof(float) * (size_t)n_views * fe::kSilBlocksPerView);
+  float* cnt_part = reinterpret_cast<float*>(base + sil_align(sizeof(fe_sil_view) * (size_t)n_views));
+  float* loss_part = reinterpret_cast<float*>(reinterpret_cast<char*>(cnt_part) + part);
+  if (!(flags & FE_FLAG_REUSE_DESCRIPTORS)) {
+    fe_status st = fe::upload(s, d_views, views, sizeof(fe_sil_view) * (size_t)n_views);
+    if (st != FE_OK) return st;
+  }
+  FE_HIP(fe::launch_sil_loss(s, d_views, n_views, cnt_part, loss_part, w_depth, clip, loss));
+  return FE_OK;
+}
+
+fe_status fe_adam_step(void* stream, int64_t n, float* param, const float* grad, float* m, float* v, float lr,
+                       float beta1, float beta2, float eps, int32_t step) {
+  if (n < 0 || (n > 0 && (!param || !grad || !m || !v)) || step < 1 || !(beta1 >= 0.0f && beta1 < 1.0f) ||
+      !(beta2 >= 0.0f && beta2 < 1.0f) || !(eps > 0.0f)) {
+    fe::set_msg("fe_adam_step: need buffers, step >= 1, 0 <= beta < 1 and eps > 0");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  const float c1 = (float)(1.0 - std::pow((double)beta1, (double)step));
+  const float c2 = (float)(1.0 - std::pow((double)beta2, (double)step));
+  FE_HIP(fe::launch_adam(reinterpret_cast<hipStream_t>(stream), n, param, grad, m, v, lr, beta1, beta2, c1, c2,
+                         eps));
+  return FE_OK;
 }
 
 }  // extern "C"

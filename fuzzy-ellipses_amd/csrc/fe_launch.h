@@ -4,10 +4,15 @@
 #include <hip/hip_runtime_api.h>
 #include <stdint.h>
 
+#include "../../include/fuzzy_ellipses_amd.h"
+
 namespace fe {
 
 struct Job;
 struct Counters;
+
+// Blocks per view of the silhouette objective kernels (partial sums per view).
+constexpr int kSilBlocksPerView = 64;
 
 hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs);
 hipError_t launch_pose(hipStream_t s, Job* jobs, int n_jobs);
@@ -33,5 +38,10 @@ hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Jo
                             int n_jobs, const float* rec, const unsigned long long* off1,
                             const unsigned long long* off2, const uint32_t* list, const float* st,
                             int64_t total_pix, const int32_t* job_flags, int dense);
+
+hipError_t launch_sil_loss(hipStream_t s, const fe_sil_view* views, int n_views, float* cnt_part,
+                           float* loss_part, float w_depth, float clip, float* loss);
+hipError_t launch_adam(hipStream_t s, int64_t n, float* p, const float* g, float* m, float* v, float lr,
+                       float b1, float b2, float c1, float c2, float eps);
 
 }  // namespace fe

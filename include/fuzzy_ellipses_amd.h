@@ -182,6 +182,41 @@ fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float
                         float beta3, float* depth, float* alpha, void* workspace,
                         size_t workspace_bytes);
 
+/* ---- shape-from-silhouette caller: objective + cotangents, Adam ----
+ * The optimisation loop that calls fe_forward / fe_backward each step
+ * (shape_from_silhouette.ipynb: "jax_objective" = mean BCE of alpha clipped to
+ * [3e-8, 1 - 3e-8] against the target silhouette; optimizers.adam; DegradeLR schedule in
+ * utils/sfs_utils.py:62-131, which stays on the host).  Here the objective of one view is
+ *   sum_p BCE(clip(alpha), mask) / P + w_depth * sum_p mask |depth - target_depth| / max(sum mask, 1)
+ * and the loss is the mean over views.  The cotangents written are d(loss)/d(alpha) (zero where
+ * the clip is active) and d(loss)/d(depth) (sign(0) = 0), ready for fe_backward. */
+typedef struct fe_sil_view {
+  const float* alpha;         /* (P,) rendered alpha of the view */
+  const float* depth;         /* (P,) rendered depth */
+  const float* mask;          /* (P,) target silhouette, values in [0, 1] */
+  const float* target_depth;  /* (P,) */
+  float* d_alpha;             /* (P,) written */
+  float* d_depth;             /* (P,) written */
+  int64_t n_pixels;           /* P > 0 */
+} fe_sil_view;
+
+/* Workspace bytes of fe_silhouette_loss for n_views views (descriptors and partial sums). */
+fe_status fe_silhouette_workspace_bytes(int32_t n_views, size_t* bytes);
+
+/* Writes the cotangents of every view and the scalar loss (device float, overwritten).
+ * The sums run in a fixed order: the loss is bitwise reproducible.  `flags` may hold
+ * FE_FLAG_REUSE_DESCRIPTORS (same views as the previous call on this workspace: nothing is
+ * uploaded, so the call can be captured into a hipGraph). */
+fe_status fe_silhouette_loss(void* stream, const fe_sil_view* views, int32_t n_views, float w_depth,
+                             float clip, float* loss, int32_t flags, void* workspace,
+                             size_t workspace_bytes);
+
+/* One Adam update of n float parameters in place (jax.example_libraries.optimizers.adam, as
+ * used by the notebook): m = b1 m + (1-b1) g; v = b2 v + (1-b2) g^2;
+ * p -= lr (m / (1-b1^step)) / (sqrt(v / (1-b2^step)) + eps).  step counts from 1. */
+fe_status fe_adam_step(void* stream, int64_t n, float* param, const float* grad, float* m, float* v,
+                       float lr, float beta1, float beta2, float eps, int32_t step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -125,3 +125,22 @@ def test_looser_alpha_budget_lists_fewer_pairs(fea, scenes_mod):
                                np.float32(c_alpha), np.float32(1.0), np.float32(40.0), None, c, lst)
         counts.append(tot)
     assert counts[0] >= counts[1] >= counts[2]
+
+
+def test_silhouette_and_adam_entry_points_validate(fea):
+    """The caller-side entry points validate before touching the device."""
+    from fuzzy_ellipses_amd import _lib
+    lib = _lib.load()
+    assert ctypes.sizeof(_lib.FeSilView) == 56
+    nb = ctypes.c_size_t(0)
+    assert lib.fe_silhouette_workspace_bytes(16, ctypes.byref(nb)) == 0
+    assert nb.value >= 16 * 56 + 2 * 16 * 64 * 4
+    assert lib.fe_silhouette_workspace_bytes(0, ctypes.byref(nb)) == 1
+    views = (_lib.FeSilView * 1)(_lib.FeSilView(None, None, None, None, None, None, 10))
+    assert lib.fe_silhouette_loss(None, views, 1, 0.1, 3e-8, 8, 0, None, 0) == 1      # null buffers
+    good = (_lib.FeSilView * 1)(_lib.FeSilView(8, 8, 8, 8, 8, 8, 10))
+    assert lib.fe_silhouette_loss(None, good, 1, 0.1, 0.7, 8, 0, None, 0) == 1        # clip out of range
+    assert lib.fe_silhouette_loss(None, good, 1, 0.1, 3e-8, 8, 0, None, 0) == 2       # no workspace
+    assert lib.fe_adam_step(None, 10, 8, 8, 8, 8, 1e-2, 0.9, 0.999, 1e-8, 0) == 1     # step counts from 1
+    assert lib.fe_adam_step(None, 10, 8, 8, 8, 8, 1e-2, 1.0, 0.999, 1e-8, 1) == 1     # beta1 = 1
+    assert lib.fe_adam_step(None, 0, None, None, None, None, 1e-2, 0.9, 0.999, 1e-8, 1) == 0  # empty: no-op

@@ -67,3 +67,106 @@ def test_adam_lowers_loss(fea, scenes_mod):
     print("silhouette: loss %.4f -> %.4f in %d steps (%.2f ms/step)" %
           (out["loss_first"], out["loss_last"], out["steps"], out["ms_per_step"]))
     assert out["loss_last"] < 0.9 * out["loss_first"]
+
+
+def _objective64(alpha, depth, mask, td, w_depth=0.1, clip=3e-8, n_views=1):
+    """float64 restatement of the per-view objective and its image cotangents."""
+    import torch
+    a, d, m, t = (x.double() for x in (alpha, depth, mask, td))
+    P = a.numel()
+    # the clip bounds as the float32 kernel holds them
+    lo = float(np.float32(clip))
+    hi = float(np.float32(1.0) - np.float32(clip))
+    ac = a.clamp(lo, hi)
+    bce = -(m * torch.log(ac) + (1 - m) * torch.log1p(-ac)).sum() / P
+    ms = m.sum().clamp_min(1.0)
+    diff = d - t
+    loss = (bce + w_depth * (m * diff.abs()).sum() / ms) / n_views
+    inside = (a > lo) & (a < hi)
+    da = torch.where(inside, -(m / ac - (1 - m) / (1 - ac)), torch.zeros_like(ac)) / P / n_views
+    dd = w_depth * torch.sign(diff) * m / ms / n_views
+    return loss, da, dd
+
+
+def test_objective_kernel_matches_float64(fea):
+    """fe_silhouette_loss on views of different sizes, with alphas at and beyond the clip,
+    an empty mask and exact depth ties, against float64."""
+    import ctypes
+    import torch
+    from fuzzy_ellipses_amd import _lib, _stream
+    lib = _lib.load()
+    dev = torch.device("cuda", 0)
+    g = torch.Generator().manual_seed(7)
+    sizes = [(48, 64), (17, 33), (128, 128)]
+    views, keep = [], []
+    for k, (H, W) in enumerate(sizes):
+        a = torch.rand(H, W, generator=g)
+        a.view(-1)[:6] = torch.tensor([0.0, 1.0, 3e-8, 1 - 3e-8, 1e-9, 0.5])
+        m = (torch.rand(H, W, generator=g) > 0.5).float() if k != 1 else torch.zeros(H, W)
+        d = torch.randn(H, W, generator=g)
+        t = d.clone()
+        t.view(-1)[H * W // 2:] += torch.randn(H * W - H * W // 2, generator=g)
+        a, m, d, t = (x.to(dev).contiguous() for x in (a, m, d, t))
+        da, dd = torch.full_like(a, float("nan")), torch.full_like(a, float("nan"))
+        keep.append((a, d, m, t, da, dd))
+        views.append(_lib.FeSilView(a.data_ptr(), d.data_ptr(), m.data_ptr(), t.data_ptr(), da.data_ptr(),
+                                    dd.data_ptr(), a.numel()))
+    arr = (_lib.FeSilView * len(views))(*views)
+    nb = ctypes.c_size_t(0)
+    _lib.check(lib.fe_silhouette_workspace_bytes(len(views), ctypes.byref(nb)))
+    ws = torch.empty(nb.value, dtype=torch.uint8, device=dev)
+    loss = torch.empty((), device=dev)
+    losses = []
+    for flags in (0, _lib.FE_FLAG_REUSE_DESCRIPTORS):
+        _lib.check(lib.fe_silhouette_loss(_stream(dev), arr, len(views), 0.1, 3e-8, loss.data_ptr(), flags,
+                                          ws.data_ptr(), ws.numel()), "fe_silhouette_loss")
+        torch.cuda.synchronize()
+        losses.append(float(loss))
+    assert losses[0] == losses[1], "the loss must be bitwise reproducible"
+    want = 0.0
+    for (a, d, m, t, da, dd) in keep:
+        l64, da64, dd64 = _objective64(a.cpu(), d.cpu(), m.cpu(), t.cpu(), n_views=len(views))
+        want += float(l64)
+        assert torch.allclose(da.cpu().double(), da64, rtol=1e-5, atol=1e-12), "d_alpha"
+        assert torch.allclose(dd.cpu().double(), dd64, rtol=1e-6, atol=0), "d_depth"
+    print("objective: kernel %.9g float64 %.9g" % (losses[0], want))
+    assert abs(losses[0] - want) <= 1e-5 * abs(want)
+
+
+def test_adam_kernel_matches_torch_optimizer(fea):
+    """fe_adam_step against torch.optim.Adam (float64) over three steps with a rate change."""
+    import torch
+    from fuzzy_ellipses_amd import _lib, _stream
+    lib = _lib.load()
+    dev = torch.device("cuda", 0)
+    g = torch.Generator().manual_seed(11)
+    p0 = torch.randn(1000, generator=g)
+    grads = [torch.randn(1000, generator=g) * s for s in (1.0, 1e-3, 10.0)]
+    p = p0.to(dev).clone()
+    m, v = torch.zeros_like(p), torch.zeros_like(p)
+    ref = p0.double().clone().requires_grad_(True)
+    opt = torch.optim.Adam([ref], lr=1e-2, betas=(0.9, 0.999), eps=1e-8)
+    for k, (gr, lr) in enumerate(zip(grads, (1e-2, 1e-2, 1e-3))):
+        gd = gr.to(dev).contiguous()
+        _lib.check(lib.fe_adam_step(_stream(dev), p.numel(), p.data_ptr(), gd.data_ptr(), m.data_ptr(),
+                                    v.data_ptr(), lr, 0.9, 0.999, 1e-8, k + 1), "fe_adam_step")
+        for group in opt.param_groups:
+            group["lr"] = lr
+        ref.grad = gr.double()
+        opt.step()
+    torch.cuda.synchronize()
+    err = (p.cpu().double() - ref.detach()).abs().max().item()
+    print("adam: max abs diff vs torch.optim.Adam %.3e" % err)
+    assert err <= 2e-6
+
+
+def test_schedule_drives_the_fit(fea, scenes_mod):
+    from fuzzy_ellipses_amd.silhouette import DegradeLR, SilhouetteFit
+    init, target = scenes_mod.silhouette_problem(n=128, n_views=2, H=40, W=48, f=45.0)
+    fit = SilhouetteFit(init, target, lr=1e-2)
+    sch = DegradeLR(1e-2, p_thresh=0.5, window=8, p_window=3, slope_less=-1e-4)
+    out = fit.run(steps=60, schedule=sch)
+    print("schedule: %d steps, %d drops, final lr %.1e, loss %.4f -> %.4f" %
+          (out["steps"], out["lr_drops"], out["final_lr"], out["loss_first"], out["loss_last"]))
+    assert out["final_lr"] == 1e-2 / 10 ** out["lr_drops"]
+    assert out["loss_last"] < out["loss_first"]
