@@ -200,6 +200,51 @@ def workload_serving(fea, sc, dev, rates, n_per_rate):
     return out
 
 
+GMM_FLOPS_FWD = 21      # per (point, Gaussian) pair: p (3), A p (9), |.|^2 (5), log term, online max/sum (4)
+GMM_FLOPS_BWD = 38      # recompute (18), responsibility (2), moments (18)
+
+
+def workload_gmm(fea, sc, dev, steps):
+    """GMM point-cloud NLL, forward + backward: N=4096 Gaussians, 2^20 points."""
+    import numpy as np
+    import torch
+    from fuzzy_ellipses_amd.gmm import GMM
+    rng = np.random.default_rng(5)
+    means, U, wl = sc.random_gaussians(rng, 4096)
+    g = GMM(means, U, wl, device=dev)
+    m = 1 << 20
+    pts = torch.tensor(rng.uniform(-0.6, 0.6, size=(m, 3)), dtype=torch.float32, device=dev)
+    lse = torch.empty(m, dtype=torch.float32, device=dev)
+    nll = torch.empty((), dtype=torch.float32, device=dev)
+    grads = (torch.zeros_like(g.means), torch.zeros_like(g.prec), torch.zeros_like(g.weights_log))
+    s = torch.cuda.current_stream(dev)
+    for _ in range(2):
+        g.forward(pts, lse=lse, nll=nll)
+        g.backward(1.0, grads)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tf = tb = 0.0
+    for _ in range(steps):
+        ev[0].record(s)
+        g.forward(pts, lse=lse, nll=nll)
+        ev[1].record(s)
+        g.backward(1.0, grads)
+        ev[2].record(s)
+        torch.cuda.synchronize()
+        tf += ev[0].elapsed_time(ev[1])
+        tb += ev[1].elapsed_time(ev[2])
+    pairs = 4096 * m
+    fwd_s, bwd_s = tf / steps / 1e3, tb / steps / 1e3
+    return {"metric": "GMM point-cloud NLL fwd+bwd (point, Gaussian) pairs/s (N=4096, M=2^20)",
+            "value": round(pairs / (fwd_s + bwd_s), 1), "unit": "pairs/s", "n_gpus": 1, "steps": steps,
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "fwd_ms": round(fwd_s * 1e3, 4), "bwd_ms": round(bwd_s * 1e3, 4),
+            "fwd_tflops": round(GMM_FLOPS_FWD * pairs / fwd_s / 1e12, 3),
+            "bwd_tflops": round(GMM_FLOPS_BWD * pairs / bwd_s / 1e12, 3),
+            "peak_fp32_unpacked_tflops": FP32_PEAK_UNPACKED_TFLOPS,
+            "config": {"workload": "GMM log-likelihood of 2^20 uniform points under 4096 Gaussians"}}
+
+
 def workload_silhouette(fea, sc, dev, steps):
     from fuzzy_ellipses_amd.silhouette import SilhouetteFit
     init, target = sc.silhouette_problem()
@@ -223,7 +268,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mode", default="culled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="shell", choices=["shell", "batched", "serving", "silhouette"])
+    ap.add_argument("--workload", default="shell", choices=["shell", "batched", "serving", "silhouette", "gmm"])
     ap.add_argument("--rates", default="50,200,800")
     ap.add_argument("--requests", default="500,1000,2000")
     ap.add_argument("--no-serving", action="store_true")
@@ -259,6 +304,9 @@ def main():
         rates = [float(x) for x in args.rates.split(",")]
         nreq = [int(x) for x in args.requests.split(",")]
         print(json.dumps(workload_serving(fea, sc, dev, rates, nreq)), flush=True)
+        return
+    if args.workload == "gmm":
+        print(json.dumps(workload_gmm(fea, sc, dev, max(args.steps, 10))), flush=True)
         return
     if args.workload == "silhouette":
         print(json.dumps(workload_silhouette(fea, sc, dev, max(args.steps, 200))), flush=True)

@@ -19,6 +19,7 @@ EXPORTS = [
     "fe_version", "fe_status_string", "fe_last_error_string", "fe_plan", "fe_forward",
     "fe_backward", "fe_read_stats", "fe_render_workspace_bytes", "fe_render_fwd", "fe_profile_enable",
     "fe_profile_read", "fe_silhouette_workspace_bytes", "fe_silhouette_loss", "fe_adam_step",
+    "fe_gmm_workspace_bytes", "fe_gmm_forward", "fe_gmm_backward",
 ]
 KERNELS = ["preprocess", "mask1", "scan1", "write1", "blend1", "mask2", "scan2", "write2", "blend2",
            "blend_dense", "bwd_blend", "bwd_finalize"]
@@ -130,6 +131,16 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fe_adam_step.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                  ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_int32]
+    lib.fe_gmm_workspace_bytes.restype = ctypes.c_int
+    lib.fe_gmm_workspace_bytes.argtypes = [ctypes.c_int32, ctypes.c_int64, ctypes.POINTER(ctypes.c_size_t)]
+    lib.fe_gmm_forward.restype = ctypes.c_int
+    lib.fe_gmm_forward.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                   ctypes.c_void_p, ctypes.c_size_t]
+    lib.fe_gmm_backward.restype = ctypes.c_int
+    lib.fe_gmm_backward.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
     _lib = lib
     return lib
 

@@ -591,4 +591,69 @@ fe_status fe_adam_step(void* stream, int64_t n, float* param, const float* grad,
   return FE_OK;
 }
 
+namespace {
+struct GmmLayout {
+  size_t rec, wnorm, fpart, bpart, bytes;
+};
+bool gmm_layout(int32_t n, int64_t m, GmmLayout* L) {
+  if (n < 1 || n > (1 << 24) || m < 1 || m > ((int64_t)1 << 36)) return false;
+  const int64_t chunk = fe::gmm_chunk(m);
+  const int64_t n_chunks = (m + chunk - 1) / chunk;
+  const int64_t nb = (m + 255) / 256;
+  L->rec = 0;
+  L->wnorm = sil_align(sizeof(float) * 16 * (size_t)n);
+  L->fpart = L->wnorm + 256;
+  L->bpart = L->fpart + sil_align(sizeof(float) * (size_t)nb);
+  L->bytes = L->bpart + sil_align(sizeof(float) * 10 * (size_t)n * (size_t)n_chunks);
+  return true;
+}
+}  // namespace
+
+fe_status fe_gmm_workspace_bytes(int32_t n, int64_t m, size_t* bytes) {
+  GmmLayout L;
+  if (!bytes || !gmm_layout(n, m, &L)) return FE_ERR_INVALID_ARGUMENT;
+  *bytes = L.bytes;
+  return FE_OK;
+}
+
+fe_status fe_gmm_forward(void* stream, int32_t n, const float* means, const float* prec, const float* weights_log,
+                         int64_t m, const float* points, float* point_lse, float* res, float* nll, void* workspace,
+                         size_t workspace_bytes) {
+  GmmLayout L;
+  if (!gmm_layout(n, m, &L) || !means || !prec || !weights_log || !points || !point_lse || !nll) {
+    fe::set_msg("fe_gmm_forward: need 1 <= n <= 2^24, 1 <= m <= 2^36 and non-null buffers (res may be NULL)");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  if (!workspace || workspace_bytes < L.bytes) {
+    fe::set_msg("fe_gmm_forward: workspace missing or too small (see fe_gmm_workspace_bytes)");
+    return FE_ERR_WORKSPACE;
+  }
+  char* b = reinterpret_cast<char*>(workspace);
+  FE_HIP(fe::launch_gmm_forward(reinterpret_cast<hipStream_t>(stream), n, m, means, prec, weights_log, points,
+                                reinterpret_cast<float*>(b + L.rec), reinterpret_cast<float*>(b + L.wnorm),
+                                reinterpret_cast<float*>(b + L.fpart), point_lse, res, nll));
+  return FE_OK;
+}
+
+fe_status fe_gmm_backward(void* stream, int32_t n, const float* prec, const float* weights_log, int64_t m,
+                          const float* points, const float* point_lse, float d_nll, float* d_means, float* d_prec,
+                          float* d_weights_log, void* workspace, size_t workspace_bytes) {
+  GmmLayout L;
+  if (!gmm_layout(n, m, &L) || !prec || !weights_log || !points || !point_lse || !d_means || !d_prec ||
+      !d_weights_log) {
+    fe::set_msg("fe_gmm_backward: need 1 <= n <= 2^24, 1 <= m <= 2^36 and non-null buffers");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  if (!workspace || workspace_bytes < L.bytes) {
+    fe::set_msg("fe_gmm_backward: workspace missing or too small (see fe_gmm_workspace_bytes)");
+    return FE_ERR_WORKSPACE;
+  }
+  char* b = reinterpret_cast<char*>(workspace);
+  FE_HIP(fe::launch_gmm_backward(reinterpret_cast<hipStream_t>(stream), n, m, prec, weights_log, points,
+                                 reinterpret_cast<const float*>(b + L.rec),
+                                 reinterpret_cast<const float*>(b + L.wnorm), point_lse,
+                                 reinterpret_cast<float*>(b + L.bpart), d_nll, d_means, d_prec, d_weights_log));
+  return FE_OK;
+}
+
 }  // extern "C"

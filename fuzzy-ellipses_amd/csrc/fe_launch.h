@@ -44,4 +44,12 @@ hipError_t launch_sil_loss(hipStream_t s, const fe_sil_view* views, int n_views,
 hipError_t launch_adam(hipStream_t s, int64_t n, float* p, const float* g, float* m, float* v, float lr,
                        float b1, float b2, float c1, float c2, float eps);
 
+int64_t gmm_chunk(int64_t m);
+hipError_t launch_gmm_forward(hipStream_t s, int n, int64_t m, const float* means, const float* prec,
+                              const float* wlog, const float* pts, float* rec, float* wnorm, float* part,
+                              float* lse, float* res, float* nll);
+hipError_t launch_gmm_backward(hipStream_t s, int n, int64_t m, const float* prec, const float* wlog,
+                               const float* pts, const float* rec, const float* wnorm, const float* lse,
+                               float* part, float g, float* d_means, float* d_prec, float* d_wlog);
+
 }  // namespace fe

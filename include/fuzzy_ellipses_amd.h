@@ -217,6 +217,25 @@ fe_status fe_silhouette_loss(void* stream, const fe_sil_view* views, int32_t n_v
 fe_status fe_adam_step(void* stream, int64_t n, float* param, const float* grad, float* m, float* v,
                        float lr, float beta1, float beta2, float eps, int32_t step);
 
+/* ---- Gaussian-mixture log-likelihood of a point cloud ----
+ * Replaces baseline/fm_render.py:231-259 (log_likelihood(params, points)): with the mixture
+ * weights softmax(weights_log) and the precision factor's upper triangle U (U U^T = Sigma^-1),
+ *   res[i][j] = ln N(points_j; means_i, Sigma_i) + ln pi_i,   point_lse[j] = logsumexp_i res[i][j],
+ *   nll = -mean_j point_lse[j].
+ * fe_gmm_forward writes point_lse (M,), nll (device scalar) and, when res is not NULL, the
+ * full (N, M) matrix res that the reference returns.  fe_gmm_backward, given the workspace of
+ * the matching forward and its point_lse, accumulates d_nll * d(nll)/d(means, prec upper
+ * triangle, weights_log) into the gradient buffers (zero them first).  No reference
+ * counterpart for the backward: callers used jax.grad.  Both are deterministic. */
+fe_status fe_gmm_workspace_bytes(int32_t n, int64_t m, size_t* bytes);
+fe_status fe_gmm_forward(void* stream, int32_t n, const float* means, const float* prec,
+                         const float* weights_log, int64_t m, const float* points, float* point_lse,
+                         float* res, float* nll, void* workspace, size_t workspace_bytes);
+fe_status fe_gmm_backward(void* stream, int32_t n, const float* prec, const float* weights_log,
+                          int64_t m, const float* points, const float* point_lse, float d_nll,
+                          float* d_means, float* d_prec, float* d_weights_log, void* workspace,
+                          size_t workspace_bytes);
+
 #ifdef __cplusplus
 }
 #endif
