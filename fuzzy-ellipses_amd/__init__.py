@@ -30,7 +30,8 @@ from .scenes import BETA2, BETA3, Camera, Scene
 
 __all__ = [
     "Renderer", "SceneTensors", "render_camera", "render_func_quat", "rasterize", "quat_to_rot",
-    "prec_from_scale_quat", "FeError", "Camera", "Scene", "BETA2", "BETA3",
+    "prec_from_scale_quat", "FeError", "Camera", "Scene", "BETA2", "BETA3", "RayList",
+    "render_func_idx_quattrans", "render_func_idx_quattrans_flow",
 ]
 
 MODES = {"culled": _lib.FE_MODE_CULLED, "dense": _lib.FE_MODE_DENSE}
@@ -68,7 +69,36 @@ class SceneTensors:
         return FeScene(self.n, _ptr(self.means), _ptr(self.prec), _ptr(self.weights_log))
 
 
-def fe_camera(cam: Camera) -> FeCamera:
+class RayList:
+    """The rays of one pose for a ray-list view: camera-plane coordinates ``xy`` ((n, 2) float32
+    device tensor; ray k has camera-frame direction (xy[k, 0], xy[k, 1], 1)) and the pose
+    ``R`` (rows = camera axes in world coordinates), ``t`` (ray origin)."""
+
+    def __init__(self, R, t, xy: torch.Tensor):
+        self.R = np.asarray(R, dtype=np.float64).reshape(3, 3)
+        self.t = np.asarray(t, dtype=np.float64).reshape(3)
+        if xy.dtype != torch.float32 or xy.dim() != 2 or xy.shape[1] != 2 or not xy.is_contiguous():
+            raise ValueError("RayList: xy must be a contiguous (n, 2) float32 tensor")
+        if xy.shape[0] < 1:
+            raise ValueError("RayList: needs at least one ray")
+        self.xy = xy
+
+    @property
+    def n(self) -> int:
+        return int(self.xy.shape[0])
+
+
+def fe_camera(cam) -> FeCamera:
+    if isinstance(cam, RayList):
+        c = FeCamera()
+        R = cam.R.astype(np.float32).reshape(9)
+        for i in range(9):
+            c.R[i] = float(R[i])
+        for i in range(3):
+            c.t[i] = float(np.float32(cam.t[i]))
+        c.fx = c.fy = 1.0
+        c.height, c.width = 1, cam.n
+        return c
     c = FeCamera()
     R = np.asarray(cam.R, dtype=np.float32).reshape(9)
     t = np.asarray(cam.t, dtype=np.float32).reshape(3)
@@ -79,6 +109,13 @@ def fe_camera(cam: Camera) -> FeCamera:
     c.fx, c.fy, c.cx, c.cy = float(cam.fx), float(cam.fy), float(cam.cx), float(cam.cy)
     c.height, c.width = int(cam.H), int(cam.W)
     return c
+
+
+def _fe_view(si, cam, b2, b3, depth_ptr, alpha_ptr) -> FeView:
+    v = FeView(int(si), fe_camera(cam), float(b2), float(b3), depth_ptr, alpha_ptr)
+    if isinstance(cam, RayList):
+        v.rays, v.n_rays = cam.xy.data_ptr(), cam.n
+    return v
 
 
 class Renderer:
@@ -116,7 +153,7 @@ class Renderer:
         vw = (FeView * len(views))()
         for i, (si, cam, b2, b3) in enumerate(views):
             d, a = outs[i]
-            vw[i] = FeView(int(si), fe_camera(cam), float(b2), float(b3), _ptr(d), _ptr(a))
+            vw[i] = _fe_view(si, cam, b2, b3, _ptr(d), _ptr(a))
         self._cache = (key, sc, vw)
         return sc, vw
 
@@ -124,7 +161,7 @@ class Renderer:
         sc = (FeScene * len(scenes))(*[s.fe() for s in scenes])
         vw = (FeView * len(views))()
         for i, (si, cam, b2, b3) in enumerate(views):
-            vw[i] = FeView(int(si), fe_camera(cam), float(b2), float(b3), None, None)
+            vw[i] = _fe_view(si, cam, b2, b3, None, None)
         opt = self._opt(save_for_backward)
         nbytes = ctypes.c_size_t(0)
         _lib.check(self.lib.fe_plan(sc, len(scenes), vw, len(views), ctypes.byref(opt), ctypes.byref(nbytes)),
@@ -139,16 +176,18 @@ class Renderer:
         return self._ws
 
     def alloc_outputs(self, views):
-        return [(torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device),
-                 torch.empty((cam.H, cam.W), dtype=torch.float32, device=self.device))
+        shape = lambda cam: (cam.n,) if isinstance(cam, RayList) else (cam.H, cam.W)
+        return [(torch.empty(shape(cam), dtype=torch.float32, device=self.device),
+                 torch.empty(shape(cam), dtype=torch.float32, device=self.device))
                 for (_, cam, _, _) in views]
 
     # -- forward ------------------------------------------------------------------------
     def forward(self, scenes: Sequence[SceneTensors], views, save_for_backward: bool = False,
                 outs=None, reuse_descriptors: bool = False) -> List[Tuple[torch.Tensor, torch.Tensor]]:
-        """Render every view.  ``views`` = [(scene_index, Camera, beta2, beta3), ...].
+        """Render every view.  ``views`` = [(scene_index, Camera or RayList, beta2, beta3), ...].
 
-        Returns [(depth (H,W), alpha (H,W)), ...] float32 device tensors.  With
+        Returns [(depth, alpha), ...] float32 device tensors, (H, W) for a Camera and (n,)
+        for a RayList.  With
         ``reuse_descriptors`` (same scenes, views, outputs and options as the previous call)
         the call only enqueues kernels and may be captured into a HIP graph.
         """
@@ -307,3 +346,6 @@ def rasterize(means3D, scales, rotations, weights, camera_rays, camera_rot, came
     d = torch.nan_to_num(d, nan=0.0, posinf=10.0, neginf=0.0)
     a = torch.nan_to_num(a, nan=0.0, posinf=1.0, neginf=0.0)
     return d, a
+
+
+from .rays import render_func_idx_quattrans, render_func_idx_quattrans_flow  # noqa: E402

@@ -19,7 +19,7 @@ EXPORTS = [
     "fe_version", "fe_status_string", "fe_last_error_string", "fe_plan", "fe_forward",
     "fe_backward", "fe_read_stats", "fe_render_workspace_bytes", "fe_render_fwd", "fe_profile_enable",
     "fe_profile_read", "fe_silhouette_workspace_bytes", "fe_silhouette_loss", "fe_adam_step",
-    "fe_gmm_workspace_bytes", "fe_gmm_forward", "fe_gmm_backward",
+    "fe_gmm_workspace_bytes", "fe_gmm_forward", "fe_gmm_backward", "fe_ray_flow",
 ]
 KERNELS = ["preprocess", "mask1", "scan1", "write1", "blend1", "mask2", "scan2", "write2", "blend2",
            "blend_dense", "bwd_blend", "bwd_finalize"]
@@ -39,7 +39,7 @@ class FeScene(ctypes.Structure):
 class FeView(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_int32), ("cam", FeCamera), ("beta2", ctypes.c_float),
                 ("beta3", ctypes.c_float), ("depth", ctypes.c_void_p), ("alpha", ctypes.c_void_p),
-                ("pose_device", ctypes.c_void_p)]
+                ("pose_device", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("n_rays", ctypes.c_int64)]
 
 
 class FeSceneGrad(ctypes.Structure):
@@ -141,6 +141,9 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fe_gmm_backward.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_void_p,
                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t]
+    lib.fe_ray_flow.restype = ctypes.c_int
+    lib.fe_ray_flow.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
+                                ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
     _lib = lib
     return lib
 

@@ -111,8 +111,8 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   const BwdJob& bj = A.bjobs[ji];
   const int tile = (int)(gt - jb.tile_base);
   const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
-  const int wv = min(TS, jb.W - tx * TS), hv = min(TS, jb.H - ty * TS);   // valid extent
-  const int npx = wv * hv;
+  const int wv = min(TS, jb.W - tx * TS);   // valid width (a ray-list tile is TS wide)
+  const int npx = tile_valid(jb, tile, tx, ty, TS);
 
   int64_t s1, n1, s2, n2;
   if (A.dense) { s1 = 0; n1 = jb.n; s2 = 0; n2 = 0; }
@@ -139,7 +139,9 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     float cD = (den > 0.0f) ? gD / den : 0.0f;
     if (!(cE == cE)) cE = 0.0f;
     if (!(cD == cD) || !(D == D)) cD = 0.0f;
-    pxa[i] = make_float4((float)px - jb.cx, (float)py - jb.cy, M, D);
+    const float xr = jb.ray_xy ? jb.ray_xy[2 * lp] : (float)px - jb.cx;
+    const float yr = jb.ray_xy ? jb.ray_xy[2 * lp + 1] : (float)py - jb.cy;
+    pxa[i] = make_float4(xr, yr, M, D);
     pxb[i] = make_float2(cE, cD);
   }
   __syncthreads();

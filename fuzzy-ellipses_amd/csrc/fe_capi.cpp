@@ -42,12 +42,13 @@ static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 struct Layout {
   int n_jobs = 0;
   int max_n = 0;
+  int has_rays = 0;
   int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0, reuse = 0;
   int64_t total_stiles = 0;
   int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
-         o_bjobs = 0, bytes = 0;
+         o_bjobs = 0, o_rect = 0, bytes = 0;
 };
 
 // A small header at the start of the workspace lets fe_read_stats find the counters.
@@ -105,15 +106,29 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     }
     const fe_scene& sc = scenes[vw.scene];
     const fe_camera& c = vw.cam;
-    if (sc.n < 0 || c.height <= 0 || c.width <= 0 || !(c.fx != 0.0f) || !(c.fy != 0.0f)) {
+    const bool ray_view = vw.rays != nullptr;
+    if (ray_view) {
+      if (sc.n < 0 || vw.n_rays <= 0 || vw.n_rays > ((int64_t)1 << 30)) {
+        set_msg("fe_plan: a ray-list view needs 1 <= n_rays <= 2^30");
+        return FE_ERR_INVALID_ARGUMENT;
+      }
+    } else if (sc.n < 0 || c.height <= 0 || c.width <= 0 || !(c.fx != 0.0f) || !(c.fy != 0.0f)) {
       set_msg("fe_plan: invalid scene size or camera");
       return FE_ERR_INVALID_ARGUMENT;
     }
-    if ((int64_t)c.height * c.width > (int64_t)1 << 30 || sc.n > (1 << 26)) {
+    if ((!ray_view && (int64_t)c.height * c.width > (int64_t)1 << 30) || sc.n > (1 << 26)) {
       set_msg("fe_plan: image or scene too large");
       return FE_ERR_INVALID_ARGUMENT;
     }
-    const int tx = (c.width + o.tile - 1) / o.tile, ty = (c.height + o.tile - 1) / o.tile;
+    // a ray list is laid out as an image one tile wide: tile t holds rays [t*ts*ts, (t+1)*ts*ts)
+    const int64_t tt = (int64_t)o.tile * o.tile;
+    const int Wv = ray_view ? o.tile : c.width;
+    const int Hv = ray_view ? (int)((vw.n_rays + o.tile - 1) / o.tile) : c.height;
+    const int64_t npix = ray_view ? vw.n_rays : (int64_t)c.height * c.width;
+    const int tx = ray_view ? 1 : (c.width + o.tile - 1) / o.tile;
+    const int ty = ray_view ? (int)((vw.n_rays + tt - 1) / tt) : (c.height + o.tile - 1) / o.tile;
+    const int sfv = ray_view ? 1 : sf;
+    if (ray_view) L->has_rays = 1;
     const int nw = (sc.n + 63) / 64;
     if (jobs) {
       Job& jb = (*jobs)[v];
@@ -126,7 +141,13 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       jb.pose = vw.pose_device;
       memcpy(jb.R, c.R, sizeof(jb.R));
       memcpy(jb.o, c.t, sizeof(jb.o));
-      jb.fx = c.fx; jb.fy = c.fy; jb.cx = c.cx; jb.cy = c.cy;
+      if (ray_view) {
+        jb.fx = 1.0f; jb.fy = 1.0f; jb.cx = 0.0f; jb.cy = 0.0f;
+        jb.ray_xy = vw.rays;
+      } else {
+        jb.fx = c.fx; jb.fy = c.fy; jb.cx = c.cx; jb.cy = c.cy;
+      }
+      jb.npix = npix;
       jb.beta2 = vw.beta2; jb.beta3 = vw.beta3;
       const double ln_n = log((double)std::max(sc.n, 1));
       // alpha: total dropped density <= n * exp(-c_alpha) = alpha_eps
@@ -134,12 +155,12 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       // depth: dropped softmax weight fraction <= n * exp(-delta) = depth_eps
       jb.delta2 = (float)((ln_n - log((double)o.depth_eps)) * 1.4426950408889634);
       jb.floor_thr = (float)(13.0 * 3.321928094887362 - ln_n * 1.4426950408889634);
-      jb.n = sc.n; jb.H = c.height; jb.W = c.width;
+      jb.n = sc.n; jb.H = Hv; jb.W = Wv;
       jb.tiles_x = tx; jb.tiles_y = ty; jb.n_tiles = tx * ty;
       jb.nw = nw;
-      jb.sf = sf;
-      jb.stiles_x = (tx + sf - 1) / sf;
-      jb.stiles_y = (ty + sf - 1) / sf;
+      jb.sf = sfv;
+      jb.stiles_x = (tx + sfv - 1) / sfv;
+      jb.stiles_y = (ty + sfv - 1) / sfv;
       jb.stile_base = stiles;
       jb.scene = vw.scene;
       jb.rec_base = rec;
@@ -150,9 +171,9 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     max_n = std::max(max_n, sc.n);
     rec += sc.n;
     tiles += (int64_t)tx * ty;
-    stiles += (int64_t)((tx + sf - 1) / sf) * ((ty + sf - 1) / sf);
+    stiles += (int64_t)((tx + sfv - 1) / sfv) * ((ty + sfv - 1) / sfv);
     maskw += (int64_t)tx * ty * nw;
-    pix += (int64_t)c.height * c.width;
+    pix += npix;
     cap += (int64_t)tx * ty * sc.n;
   }
   if (cap > ((int64_t)1 << 34)) {
@@ -186,6 +207,9 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   }
   if (L->save) {
     L->o_bjobs = p; p = align256(p + sizeof(BwdJob) * n_views);
+  }
+  if (L->has_rays) {
+    L->o_rect = p; p = align256(p + sizeof(float) * 4 * tiles);
   }
   L->bytes = p;
   return FE_OK;
@@ -298,6 +322,8 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
       return FE_ERR_INVALID_ARGUMENT;
     }
   }
+  for (int v = 0; v < n_views; ++v)
+    if (jobs[v].ray_xy) jobs[v].rect = at<float>(ws, L.o_rect) + 4 * jobs[v].tile_base;
   // header + jobs in one upload
   std::vector<char> blob(L.o_counters);
   WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode,
@@ -321,6 +347,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   for (int v = 0; v < n_views; ++v) any_pose = any_pose || views[v].pose_device != nullptr;
   if (any_pose) FE_HIP(launch_pose(s, at<Job>(ws, L.o_jobs), L.n_jobs));
   FE_HIP(timed(s, FE_K_PREPROCESS, [&] { return launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, flags); }));
+  if (L.has_rays) FE_HIP(launch_ray_rects(s, dj, L.n_jobs, L.total_tiles, L.ts, flags));
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
       return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, nullptr, nullptr, nullptr, state,
@@ -522,6 +549,7 @@ fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float
   if (!cam) return FE_ERR_INVALID_ARGUMENT;
   fe_scene sc{n, means, prec, weights_log};
   fe_view v;
+  memset(&v, 0, sizeof(v));   // no device pose, no ray list
   v.scene = 0;
   v.cam = *cam;
   v.beta2 = beta2;
@@ -653,6 +681,18 @@ fe_status fe_gmm_backward(void* stream, int32_t n, const float* prec, const floa
                                  reinterpret_cast<const float*>(b + L.rec),
                                  reinterpret_cast<const float*>(b + L.wnorm), point_lse,
                                  reinterpret_cast<float*>(b + L.bpart), d_nll, d_means, d_prec, d_weights_log));
+  return FE_OK;
+}
+
+fe_status fe_ray_flow(void* stream, int64_t n_rays, const float* pixel_posei, float inv_f, int32_t n_poses,
+                      const float* rot, const float* trans, const float* depth, float* flow_next, float* flow_prev) {
+  if (n_rays < 0 || n_poses < 1 || !(inv_f != 0.0f) ||
+      (n_rays > 0 && (!pixel_posei || !rot || !trans || !depth || !flow_next || !flow_prev))) {
+    fe::set_msg("fe_ray_flow: need n_rays >= 0, n_poses >= 1, inv_f != 0 and non-null buffers");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  FE_HIP(fe::launch_ray_flow(reinterpret_cast<hipStream_t>(stream), n_rays, pixel_posei, inv_f, n_poses, rot, trans,
+                             depth, flow_next, flow_prev));
   return FE_OK;
 }
 

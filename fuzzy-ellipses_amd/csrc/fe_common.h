@@ -50,6 +50,13 @@ struct Job {
   int64_t tile_base;  // first global tile
   int64_t mask_base;  // first mask word
   int64_t pix_base;   // first pixel of this view in the per-pixel state arrays
+  // Ray-list views (ray_xy != nullptr): pixel k of the view is ray k, with camera-plane
+  // coordinates (ray_xy[2k], ray_xy[2k+1]) in place of (x - cx, y - cy); the view is laid
+  // out as an image TS pixels wide, so tile t holds rays [t*TS*TS, (t+1)*TS*TS), and each
+  // tile's bounding rectangle (centre, half-extent) is rect[4*t .. 4*t+3].
+  const float* ray_xy;
+  float* rect;
+  int64_t npix;       // valid pixels: H*W, or the number of rays
 };
 
 // Per-view backward buffers: upstream gradients of the images and the scene's gradients.
@@ -100,6 +107,12 @@ struct TileGeo {
 };
 
 __device__ __forceinline__ TileGeo tile_geo_xy(const Job& jb, int tx, int ty, int ts) {
+  if (jb.ray_xy) {   // ray lists use sf = 1, so (tx, ty) is always a tile
+    const float4 r = reinterpret_cast<const float4*>(jb.rect)[(int64_t)ty * jb.tiles_x + tx];
+    TileGeo g;
+    g.Xc = r.x; g.Yc = r.y; g.hx = r.z; g.hy = r.w;
+    return g;
+  }
   int x0 = tx * ts, y0 = ty * ts;
   int xl = min(x0 + ts, jb.W) - 1, yl = min(y0 + ts, jb.H) - 1;
   TileGeo g;
@@ -108,6 +121,25 @@ __device__ __forceinline__ TileGeo tile_geo_xy(const Job& jb, int tx, int ty, in
   g.hx = (float)(xl - x0) * 0.5f;
   g.hy = (float)(yl - y0) * 0.5f;
   return g;
+}
+
+// Valid pixels (or rays) of tile (tx, ty).
+__device__ __forceinline__ int tile_valid(const Job& jb, int tile, int tx, int ty, int TS) {
+  if (jb.ray_xy) {
+    const int64_t left = jb.npix - (int64_t)tile * TS * TS;
+    return (int)(left < TS * TS ? left : TS * TS);
+  }
+  return min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS);
+}
+
+// Bounding rectangle of a ray-list tile, in the units of the camera plane, widened by
+// 2^-20 of the largest coordinate so that the rounded centre +- half-extent still covers
+// every ray.
+__device__ __forceinline__ float4 ray_rect(float xmin, float xmax, float ymin, float ymax) {
+  const float mx = fmaxf(fabsf(xmin), fabsf(xmax)) * 9.5367431640625e-07f;
+  const float my = fmaxf(fabsf(ymin), fabsf(ymax)) * 9.5367431640625e-07f;
+  return make_float4(0.5f * (xmin + xmax), 0.5f * (ymin + ymax), 0.5f * (xmax - xmin) + mx,
+                     0.5f * (ymax - ymin) + my);
 }
 
 // Conservative bounds of one Gaussian over one tile's pixel rays.

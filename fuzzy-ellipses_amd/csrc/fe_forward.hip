@@ -326,6 +326,45 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
 }
 
 // ------------------------------------------------------------------------------------------
+// ray-list views: bounding rectangle of each tile's rays (one block per tile)
+// ------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_ray_rects(const Job* __restrict__ jobs, int n_jobs, int ts,
+                                                   int32_t* __restrict__ job_flags) {
+  __shared__ float sh[4][4];
+  const int64_t gt = blockIdx.x;
+  const int ji = find_job(n_jobs, gt, [&](int k) { return jobs[k].tile_base; });
+  const Job& jb = jobs[ji];
+  if (!jb.ray_xy) return;   // block-uniform
+  const int tile = (int)(gt - jb.tile_base);
+  const int nv = tile_valid(jb, tile, 0, tile, ts);
+  const int64_t k0 = (int64_t)tile * ts * ts;
+  float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+  bool bad = false;
+  for (int i = threadIdx.x; i < nv; i += 256) {
+    const float x = jb.ray_xy[2 * (k0 + i)], y = jb.ray_xy[2 * (k0 + i) + 1];
+    bad |= !(fabsf(x) <= 3.0e38f) || !(fabsf(y) <= 3.0e38f);
+    xmin = fminf(xmin, x); xmax = fmaxf(xmax, x);
+    ymin = fminf(ymin, y); ymax = fmaxf(ymax, y);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    xmin = fminf(xmin, __shfl_xor(xmin, o, 64)); xmax = fmaxf(xmax, __shfl_xor(xmax, o, 64));
+    ymin = fminf(ymin, __shfl_xor(ymin, o, 64)); ymax = fmaxf(ymax, __shfl_xor(ymax, o, 64));
+  }
+  const int wave = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sh[wave][0] = xmin; sh[wave][1] = xmax; sh[wave][2] = ymin; sh[wave][3] = ymax; }
+  const int any_bad = __syncthreads_or(bad ? 1 : 0);
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      xmin = fminf(xmin, sh[w][0]); xmax = fmaxf(xmax, sh[w][1]);
+      ymin = fminf(ymin, sh[w][2]); ymax = fmaxf(ymax, sh[w][3]);
+    }
+    reinterpret_cast<float4*>(jb.rect)[tile] = ray_rect(xmin, xmax, ymin, ymax);
+    // a non-finite ray makes the view dense with the NaN-safe blend, like degenerate records
+    if (any_bad) atomicOr(&job_flags[ji], 1);
+  }
+}
+
+// ------------------------------------------------------------------------------------------
 // exclusive prefix sum: uint32 counts -> uint64 offsets (off[n] = base + total)
 // ------------------------------------------------------------------------------------------
 constexpr int kScanBlock = 1024;  // items per block (256 threads x 4)
@@ -487,6 +526,7 @@ __device__ __forceinline__ void blend_batch(const float4* lds, int nbr, float xf
     for (int s = 0; s < SUB; ++s) {
       float sv = __builtin_amdgcn_exp2f(ls[s] - M);
       sv = (tsv[s] > 0.0f) ? sv : 0.0f;
+      if (SAFE) sv = (sv == sv) ? sv : 0.0f;   // an infinite logit: exp(inf - inf) -> nan_to_num -> 0
       S += sv;
       Z = fmaf(sv, tsv[s], Z);
     }
@@ -523,10 +563,17 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
   const int px = tx * TS + (threadIdx.x % TS);
   const int py = ty * TS + (threadIdx.x / TS);
-  const bool inimg = (px < jb.W) && (py < jb.H);
-  const int64_t pix = jb.pix_base + (int64_t)(inimg ? py : 0) * jb.W + (inimg ? px : 0);
-  const float xf = (float)px - jb.cx;
-  const float yf = (float)py - jb.cy;
+  const int64_t lp = (int64_t)py * jb.W + px;
+  const bool inimg = (px < jb.W) && (py < jb.H) && (lp < jb.npix);
+  const int64_t pix = jb.pix_base + (inimg ? lp : 0);
+  float xf, yf;
+  if (jb.ray_xy) {
+    xf = inimg ? jb.ray_xy[2 * lp] : 0.0f;
+    yf = inimg ? jb.ray_xy[2 * lp + 1] : 0.0f;
+  } else {
+    xf = (float)px - jb.cx;
+    yf = (float)py - jb.cy;
+  }
   const float b3 = jb.beta3;
   const float nb2l = -jb.beta2 * kLog2e;
   const float kd = -0.5f * kLog2e;
@@ -542,7 +589,8 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   } else {
     M = -3.402823466e38f; S = 0.f; Z = 0.f; E = 0.f; Zall = 0.f;
   }
-  bool nanf = false;
+  // a NaN logit seen by the phase-1 pass (kept in the sixth state plane between the passes)
+  bool nanf = (PHASE == 2) ? (A.st[5 * A.total_pix + pix] != 0.0f) : false;
 
   int64_t start, end;
   if (PHASE == 0) { start = 0; end = jb.n; }
@@ -580,6 +628,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
       A.st[2 * A.total_pix + pix] = Z;
       A.st[3 * A.total_pix + pix] = E;
       A.st[4 * A.total_pix + pix] = Zall;
+      A.st[5 * A.total_pix + pix] = nanf ? 1.0f : 0.0f;
     }
     // exact per-tile minimum of log2 sum exp of the unmasked listed logits; a pixel whose
     // phase-1 listed logits are NaN gets -inf (its tile then lists everything in phase 2)
@@ -589,7 +638,7 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
     if (threadIdx.x == 0) {
       A.tile_lmin[gt] = lm;
       const unsigned long long ev = (unsigned long long)(end - start) *
-                                    (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS));
+                                    (unsigned long long)tile_valid(jb, tile, tx, ty, TS);
       atomicAdd(&A.cnt->evals, ev);
       atomicAdd(&A.cnt->evals1, ev);
     }
@@ -622,17 +671,18 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
     // behind the camera pushes the true max logit far above M
     const float ubm = A.tile_ubm[gt];
     const float lhs = -43.185f + __log2f((float)max(jb.n, 1)) + fmaxf(0.0f, ubm - M);
-    const bool fl = inimg && !(lhs < ((S > 0.0f) ? __log2f(S) : -INFINITY));
+    // (a view with degenerate records lists every pair, so nothing was dropped there)
+    const bool fl = inimg && !safe && !(lhs < ((S > 0.0f) ? __log2f(S) : -INFINITY));
     const int any = __syncthreads_or(fl ? 1 : 0);
     if (threadIdx.x == 0) {
       if (any) atomicAdd(&A.cnt->floor_unresolved, 1ull);
       atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
-                                   (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS)));
+                                   (unsigned long long)tile_valid(jb, tile, tx, ty, TS));
     }
   } else {
     if (threadIdx.x == 0) {
       const unsigned long long ev = (unsigned long long)(end - start) *
-                                    (unsigned long long)(min(TS, jb.W - tx * TS) * min(TS, jb.H - ty * TS));
+                                    (unsigned long long)tile_valid(jb, tile, tx, ty, TS);
       atomicAdd(&A.cnt->evals, ev);
       atomicAdd(&A.cnt->evals1, ev);
     }
@@ -656,6 +706,13 @@ hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_j
 
 hipError_t launch_pose(hipStream_t s, Job* jobs, int n_jobs) {
   hipLaunchKernelGGL(k_pose, dim3((n_jobs + 63) / 64), dim3(64), 0, s, jobs, n_jobs);
+  return hipGetLastError();
+}
+
+hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t total_tiles, int ts,
+                            int32_t* job_flags) {
+  if (total_tiles <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ray_rects, dim3((unsigned)total_tiles), dim3(256), 0, s, jobs, n_jobs, ts, job_flags);
   return hipGetLastError();
 }
 

@@ -16,6 +16,8 @@ constexpr int kSilBlocksPerView = 64;
 
 hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs);
 hipError_t launch_pose(hipStream_t s, Job* jobs, int n_jobs);
+hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t total_tiles, int ts,
+                            int32_t* job_flags);
 hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec,
                              int32_t* job_flags);
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
@@ -51,5 +53,8 @@ hipError_t launch_gmm_forward(hipStream_t s, int n, int64_t m, const float* mean
 hipError_t launch_gmm_backward(hipStream_t s, int n, int64_t m, const float* prec, const float* wlog,
                                const float* pts, const float* rec, const float* wnorm, const float* lse,
                                float* part, float g, float* d_means, float* d_prec, float* d_wlog);
+
+hipError_t launch_ray_flow(hipStream_t s, int64_t n, const float* pp, float invF, int n_poses, const float* rot,
+                           const float* trans, const float* depth, float* fnext, float* fprev);
 
 }  // namespace fe

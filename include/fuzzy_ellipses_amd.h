@@ -72,6 +72,15 @@ typedef struct fe_view {
   const float* pose_device; /* optional device array R[9] (row-major) then t[3]; when set it
                                replaces cam.R / cam.t, read on the device (the reference
                                passed R and t as device buffers, rasterize_main.cu:60-61) */
+  /* Optional ray list (NULL = the pinhole pixel grid).  When set, the view renders n_rays
+   * rays whose camera-frame directions are (rays[2k], rays[2k+1], 1), rotated by the pose
+   * (world ray = camera ray @ R) and starting at t -- the per-ray form of
+   * baseline/fm_render.py:151-166 (render_func_idx_quattrans) for the rays of one pose.
+   * cam.fx/fy/cx/cy/height/width are then ignored and depth/alpha hold n_rays values.
+   * Consecutive groups of tile*tile rays share one culling cone, so rays ordered by image
+   * position cull best; any order gives the same images to within the culling budgets. */
+  const float* rays;
+  int64_t n_rays;
 } fe_view;
 
 /* Gradient buffers of one scene (device, accumulated into: zero them first). */
@@ -235,6 +244,19 @@ fe_status fe_gmm_backward(void* stream, int32_t n, const float* prec, const floa
                           int64_t m, const float* points, const float* point_lse, float d_nll,
                           float* d_means, float* d_prec, float* d_weights_log, void* workspace,
                           size_t workspace_bytes);
+
+/* ---- multi-pose ray batches: optical flow ----
+ * Replaces the flow half of baseline/fm_render.py:170-224 (render_func_idx_quattrans_flow).
+ * pixel_posei (n_rays, 4) holds (x, y, z, pose index) per ray, the camera-frame direction
+ * being (x invF, y invF, z); depth (n_rays,) is the rendered depth of each ray (ray parameter of
+ * the surface point, as render_func_rays returns it); rot (n_poses, 3, 3) row-major and
+ * trans (n_poses, 3) are the poses (world ray = camera ray @ R, origin t).  Each ray's point
+ * is projected into the next and the previous pose (indices clamped to [0, n_poses-1]) and
+ * flow = (x, y) - projection is written to flow_next / flow_prev (n_rays, 2).  The pose index
+ * is truncated toward zero and clamped.  Device pointers; asynchronous on the stream. */
+fe_status fe_ray_flow(void* stream, int64_t n_rays, const float* pixel_posei, float inv_f,
+                      int32_t n_poses, const float* rot, const float* trans, const float* depth,
+                      float* flow_next, float* flow_prev);
 
 #ifdef __cplusplus
 }

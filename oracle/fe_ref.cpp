@@ -291,11 +291,14 @@ static Geo geo_of(int tx, int ty, int ts, int W, int H, float cx, float cy) {
   return g;
 }
 
-int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, int ts, float cx, float cy,
-                     float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
-                     const float* tile_lmin, uint32_t* counts, uint32_t* list) {
-  const int tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
-  const int sf = (ts == 8) ? 4 : (ts == 16 ? 2 : 1);
+}  // extern "C"
+
+// Shared body of the grid and ray-list membership: tx x ty tiles, groups of sf x sf tiles,
+// geo(txi, tyi) the tile's rectangle and ggeo(sx, sy) the group's.
+template <typename GeoT, typename GroupGeoT>
+static int64_t tiles_impl(int n, const float* rec, int any_degenerate, int tx, int ty, int sf, GeoT geo,
+                          GroupGeoT ggeo, float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
+                          const float* tile_lmin, uint32_t* counts, uint32_t* list) {
   const float b3l = beta3 * kLog2e, b2l = beta2 * kLog2e;
   int64_t pos = 0;
   std::vector<char> in1(n > 0 ? n : 1, 0);
@@ -306,7 +309,7 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
     if (!any_degenerate) {
       for (int tile = 0; tile < tx * ty; ++tile) {
         if (!(lm[tile] == -INFINITY)) continue;
-        const Geo g = geo_of(tile % tx, tile / tx, ts, W, H, cx, cy);
+        const Geo g = geo(tile % tx, tile / tx);
         float lb = -INFINITY;
         for (int j = 0; j < n; ++j) {
           const float* r = rec + 16 * (size_t)j;
@@ -324,10 +327,10 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
   }
   for (int tile = 0; tile < tx * ty; ++tile) {
     const int txi = tile % tx, tyi = tile / tx;
-    const Geo g = geo_of(txi, tyi, ts, W, H, cx, cy);
+    const Geo g = geo(txi, tyi);
     // group of this tile
     const int sx = txi / sf, sy = tyi / sf;
-    const Geo gg = geo_of(sx, sy, ts * sf, W, H, cx, cy);
+    const Geo gg = ggeo(sx, sy);
     float tau_grp = INFINITY;
     if (tile_lmin) {
       const int nsx = std::min(sf, tx - sx * sf), nsy = std::min(sf, ty - sy * sf);
@@ -392,6 +395,53 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
     counts[tile] = c;
   }
   return pos;
+}
+
+extern "C" {
+
+int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, int ts, float cx, float cy,
+                     float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
+                     const float* tile_lmin, uint32_t* counts, uint32_t* list) {
+  const int tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
+  const int sf = (ts == 8) ? 4 : (ts == 16 ? 2 : 1);
+  return tiles_impl(
+      n, rec, any_degenerate, tx, ty, sf, [&](int a, int b) { return geo_of(a, b, ts, W, H, cx, cy); },
+      [&](int a, int b) { return geo_of(a, b, ts * sf, W, H, cx, cy); }, beta2, beta3, c_alpha, delta2, floor_thr,
+      tile_lmin, counts, list);
+}
+
+// Ray-list views: tile t holds rays [t*ts*ts, (t+1)*ts*ts) of (x', y') camera-plane
+// coordinates; its rectangle is the rays' bounding box widened by 2^-20 of the largest
+// coordinate (the GPU's ray_rect, same float operations).
+void fe_ref_ray_rects(int64_t n_rays, int ts, const float* xy, float* rects) {
+  const int64_t tt = (int64_t)ts * ts;
+  const int64_t T = (n_rays + tt - 1) / tt;
+  for (int64_t t = 0; t < T; ++t) {
+    float xmin = INFINITY, xmax = -INFINITY, ymin = INFINITY, ymax = -INFINITY;
+    for (int64_t k = t * tt; k < std::min(n_rays, (t + 1) * tt); ++k) {
+      xmin = fminf(xmin, xy[2 * k]); xmax = fmaxf(xmax, xy[2 * k]);
+      ymin = fminf(ymin, xy[2 * k + 1]); ymax = fmaxf(ymax, xy[2 * k + 1]);
+    }
+    const float mx = fmaxf(fabsf(xmin), fabsf(xmax)) * 9.5367431640625e-07f;
+    const float my = fmaxf(fabsf(ymin), fabsf(ymax)) * 9.5367431640625e-07f;
+    rects[4 * t] = 0.5f * (xmin + xmax);
+    rects[4 * t + 1] = 0.5f * (ymin + ymax);
+    rects[4 * t + 2] = 0.5f * (xmax - xmin) + mx;
+    rects[4 * t + 3] = 0.5f * (ymax - ymin) + my;
+  }
+}
+
+int64_t fe_ref_tiles_rects(int n, const float* rec, int any_degenerate, int64_t n_tiles, const float* rects,
+                           float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
+                           const float* tile_lmin, uint32_t* counts, uint32_t* list) {
+  auto g = [&](int, int b) {
+    Geo o;
+    o.Xc = rects[4 * (int64_t)b]; o.Yc = rects[4 * (int64_t)b + 1];
+    o.hx = rects[4 * (int64_t)b + 2]; o.hy = rects[4 * (int64_t)b + 3];
+    return o;
+  };
+  return tiles_impl(n, rec, any_degenerate, 1, (int)n_tiles, 1, g, g, beta2, beta3, c_alpha, delta2, floor_thr,
+                    tile_lmin, counts, list);
 }
 
 }  // extern "C"

@@ -128,6 +128,43 @@ def render_camera(means, prec_full, weights_log, R, t, camera_rays, beta_2, beta
                             weights_log, beta_2, beta_3)
 
 
+def idx_quattrans_rays(pixel_posei, invF, poses):
+    """World rays and starts of baseline/fm_render.py:151-166: per ray, the camera-frame ray
+    (x invF, y invF, z) @ R(pose), starting at the pose's translation."""
+    pp = np.asarray(pixel_posei, dtype=np.float64)
+    ps = np.asarray(poses, dtype=np.float64)
+    rot = np.stack([quat_to_rot(q) for q in ps[:, :4]])
+    idx = pp[:, 3].astype(np.int64)
+    cam = pp[:, :3] * np.array([invF, invF, 1.0])
+    rays = np.einsum("rj,rjk->rk", cam, rot[idx])
+    return rays, ps[idx, 4:], rot
+
+
+def render_func_idx_quattrans(means, prec_full, weights_log, pixel_posei, invF, poses, beta_2, beta_3):
+    """(depth, alpha) of baseline/fm_render.py:151-166 (arguments passed to render_func_rays
+    in its own order)."""
+    rays, starts, _ = idx_quattrans_rays(pixel_posei, invF, poses)
+    return render_func_rays(rays, starts, means, prec_full, weights_log, beta_2, beta_3)
+
+
+def idx_quattrans_flow(pixel_posei, invF, poses, depth):
+    """(flow_next, flow_prev), each (R, 2), of baseline/fm_render.py:186-221."""
+    pp = np.asarray(pixel_posei, dtype=np.float64)
+    ps = np.asarray(poses, dtype=np.float64)
+    rot = np.stack([quat_to_rot(q) for q in ps[:, :4]])
+    P = ps.shape[0]
+    idx = pp[:, 3].astype(np.int64)
+    ip = np.minimum(idx + 1, P - 1)
+    im = np.maximum(idx - 1, 0)
+    cam = pp[:, :3] * np.array([invF, invF, 1.0]) * np.asarray(depth, dtype=np.float64)[:, None]
+    world = np.einsum("rj,rjk->rk", cam, rot[idx]) + ps[idx, 4:]
+
+    def proj(j):
+        c = np.einsum("rj,rkj->rk", world - ps[j, 4:], rot[j])
+        return -(c[:, :2] / (c[:, 2:3] * invF) - pp[:, :2])
+    return proj(ip), proj(im)
+
+
 def nan_to_num_outputs(depth, alpha):
     """Output clean-up of the reference binding (fuzzy_ellipses/__init__.py:233-234)."""
     return (np.nan_to_num(depth, posinf=10.0, neginf=0.0),

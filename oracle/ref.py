@@ -50,6 +50,12 @@ def load():
     lib.fe_ref_tiles.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_float, ctypes.c_float, ctypes.c_void_p, _u32p, _u32p]
+    lib.fe_ref_ray_rects.restype = None
+    lib.fe_ref_ray_rects.argtypes = [ctypes.c_int64, ctypes.c_int, _f32p, _f32p]
+    lib.fe_ref_tiles_rects.restype = ctypes.c_int64
+    lib.fe_ref_tiles_rects.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int64, _f32p, ctypes.c_float,
+                                       ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                       ctypes.c_void_p, _u32p, _u32p]
     _lib = lib
     return lib
 
@@ -136,4 +142,35 @@ def tiles(scene, cam, ts, tile_lmin=None):
                              cam.H, cam.W, ts, np.float32(cam.cx), np.float32(cam.cy), np.float32(scene.beta2),
                              np.float32(scene.beta3), np.float32(c_alpha), np.float32(delta2),
                              np.float32(floor_thr), lm, counts, lst)
+    return counts, lst[:total].copy()
+
+
+def ray_rects(xy, ts):
+    """(T, 4) float32 tile rectangles (centre x, centre y, half-width, half-height) of a ray list."""
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    T = (xy.shape[0] + ts * ts - 1) // (ts * ts)
+    out = np.zeros((T, 4), np.float32)
+    load().fe_ref_ray_rects(xy.shape[0], ts, xy.reshape(-1), out.reshape(-1))
+    return out
+
+
+def tiles_rays(scene, R, t, xy, ts, tile_lmin=None):
+    """Membership lists of a ray-list view (records with fx = fy = 1), as `tiles`."""
+    from types import SimpleNamespace
+    lib = load()
+    cam = SimpleNamespace(R=np.asarray(R), t=np.asarray(t), fx=1.0, fy=1.0)
+    rec, deg = records(scene, cam)
+    n = scene.n
+    rects = ray_rects(xy, ts)
+    T = rects.shape[0]
+    counts = np.zeros(T, np.uint32)
+    lst = np.zeros(max(n * T, 1), np.uint32)
+    c_alpha, delta2, floor_thr = cull_constants(n)
+    lm = None
+    if tile_lmin is not None:
+        lm_arr = np.ascontiguousarray(tile_lmin, np.float32)
+        lm = lm_arr.ctypes.data_as(ctypes.c_void_p)
+    total = lib.fe_ref_tiles_rects(n, np.ascontiguousarray(rec.reshape(-1) if n else np.zeros(16, np.float32)), deg,
+                                   T, rects.reshape(-1), np.float32(scene.beta2), np.float32(scene.beta3),
+                                   np.float32(c_alpha), np.float32(delta2), np.float32(floor_thr), lm, counts, lst)
     return counts, lst[:total].copy()

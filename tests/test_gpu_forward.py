@@ -84,3 +84,36 @@ def test_shell_culled_matches_dense_full_view(fea, scenes_mod):
                      outs["dense"][1].astype(np.float64))
     print(f"culled vs dense full view: depth rel {ed:.3e}, alpha abs {ea:.3e}")
     assert ea <= ALPHA_TOL and ed <= DEPTH_RTOL
+
+
+@pytest.mark.parametrize("mode", ["dense", "culled"])
+@pytest.mark.parametrize("kind", ["nan_mean", "zero_diag", "inf_weight"])
+def test_degenerate_inputs_follow_oracle(fea, scenes_mod, mode, kind):
+    """Degenerate Gaussians make the view dense with the oracle's NaN/Inf semantics: a NaN
+    logit anywhere leaves only the 1e-20 weight floor (depth = mean of nan_to_num(t*)), an
+    infinite logit gets weight nan_to_num(exp(inf - inf)) = 0, and alpha propagates NaN."""
+    import copy
+    import torch
+    sc = scenes_mod
+    scene = copy.deepcopy(sc.single_view_scene(seed=4, n=30, H=24, W=32, f=30.0))
+    if kind == "nan_mean":
+        scene.means[3, 1] = np.nan
+    elif kind == "zero_diag":
+        scene.prec[5, 1, 1] = 0.0
+    else:
+        scene.weights_log[7] = np.inf
+    cam = scene.cameras[0]
+    r = fea.Renderer(tile=16, batch=64, mode=mode)
+    (d, a), = r.forward([fea.SceneTensors.from_scene(scene)], [(0, cam, scene.beta2, scene.beta3)])
+    torch.cuda.synchronize()
+    od, oa = _oracle_view(sc, scene, cam)
+    d, a = d.cpu().numpy().ravel().astype(np.float64), a.cpu().numpy().ravel().astype(np.float64)
+    assert np.array_equal(np.isnan(a), np.isnan(oa)) and np.array_equal(np.isnan(d), np.isnan(od))
+    fd, fa = np.isfinite(od), np.isfinite(oa)
+    assert fd.any()
+    ed = _errors(d[fd], a[fd], od[fd], np.zeros(int(fd.sum())) + a[fd])[0]
+    ea = float(np.max(np.abs(a[fa] - oa[fa]))) if fa.any() else 0.0
+    print(f"degenerate {kind} {mode}: NaN alpha px {int(np.isnan(oa).sum())}, depth rel {ed:.2e}, alpha abs {ea:.2e}")
+    assert ea <= ALPHA_TOL and ed <= DEPTH_RTOL
+    if mode == "culled":
+        assert r.stats()["floor_unresolved"] == 0
