@@ -7,7 +7,14 @@
 // Per pixel (upstream gD = dL/dD, gA = dL/dalpha), D = Z/S, alpha = 1 - exp(-E):
 //   cE = gA exp(-E),  cD = gD / S,  s_j = [t_j>0] 2^(l2_j - M)
 //   g_d2 = cE e_j + b3 cD s_j (t_j - D)
-//   g_t  = cD s_j (1 - b2 (t_j - D))
+//   g_t  = cD s_j (1 - b2 (t_j - D)) + cD 1e-20
+// with cD = gD/(S + 1e-20 n).  The last term and one more are the 1e-20 weight floor of
+// fm_render.py:96 under reverse-mode differentiation: D = (Z + 1e-20 sum nan_to_num(t))/
+// (S + 1e-20 n) gives every pair dD/dt = 1e-20/(S + 1e-20 n), and because the weights are
+// exp(l - max l) (util_render.py:5-9) the pair holding the maximum logit also gets
+//   dL/dl_max = cX = -cD (Z - D S),
+// which is zero up to rounding unless the floor is visible (S below ~1e-11 n).  Tiles with
+// such a pixel run the argmax variant; elsewhere |cX| is below 1e-9 of the other terms.
 // and through t* = (q.u)/(u.u), d2 = w - |t* u - q|^2 / 2 (envelope: dd0/dt = 0), v = t u - q:
 //   g_q = g_d2 v + (g_t/vsv) u
 //   g_u = -g_d2 t v + (g_t/vsv) (q - 2 t u)
@@ -15,6 +22,18 @@
 // Per (tile, Gaussian) the kernel accumulates 13 sums: G_q (3), G_a = sum g_u x' (3),
 // G_b = sum g_u y' (3), G_g = sum g_u (3), G_w, then maps them through the chain rule to the
 // 10 gradient components of (mu, upper U, w) and adds those to the scene gradients.
+//
+// Views with blend records (fe_common.h) recompute each pair from the three linear forms
+// X, Y1, Y2 of p = (x', y', 1) instead (t = X r, d0 = |q|^2 W r, r = 1/(X^2 + W)):
+//   g_X  = g_t r + G t - t^2 h,   g_Yi = -c Y_i,
+//   with G = g_d2 |q|^2, h = 2 g_t + G X, c = h t r,
+// accumulating 11 sums per (tile, Gaussian): sum g_X p (3), sum c Y_i p (3 + 3), sum g_d2,
+// and sum g_d2 W r (d2 depends on |q|^2 directly: dd2/d|q|^2 = -W r / 2).  The chain rule
+// to (mu, U) goes through the blend record K = M'^T B, B = [q/|q|^2, e1/|q|, e2/|q|]
+// (M' = [a b g]): dM' = B dK^T, dB = M' dK, and for the basis the derivative that keeps
+// e1, e2 orthonormal with no rotation in their plane (W does not depend on that rotation):
+//   dq = (dB_0 - 2 n (n.dB_0))/|q|^2 - sum_i (e_i (n.dB_i) + n (e_i.dB_i))/|q|^2 - q sum g_d2 W r,
+// with n = q/|q|, once per (tile, Gaussian).
 #include "fe_common.h"
 
 #include <string.h>
@@ -28,7 +47,8 @@ namespace fe {
 //   g_u = -(g_t/vsv + g_d2 t) v - (g_t/vsv) t u.
 // SAFE zeroes non-finite per-pair terms (views with degenerate records only).
 template <bool SAFE>
-__device__ __forceinline__ void bwd_pixels(const float4* pxa, const float2* pxb, int npx, float a0, float a1,
+__device__ __forceinline__ void bwd_pixels(const float4* pxa, const float4* pxb, const float* pxc, int npx, float a0,
+                                           float a1,
                                            float a2, float bb0, float bb1, float bb2, float g0, float g1, float g2,
                                            float q0, float q1, float q2, float wl, float b3, float b2, float nb2l,
                                            float kd, float& Gq0, float& Gq1, float& Gq2, float& Ga0, float& Ga1,
@@ -36,8 +56,8 @@ __device__ __forceinline__ void bwd_pixels(const float4* pxa, const float2* pxb,
                                            float& Gg2, float& Gw) {
   for (int p = 0; p < npx; ++p) {
     const float4 pa = pxa[p];
-    const float2 pb = pxb[p];
-    const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y;
+    const float4 pb = pxb[p];
+    const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y, cF = pb.w;
     const float u0 = fmaf(a0, xf, fmaf(bb0, yf, g0));
     const float u1 = fmaf(a1, xf, fmaf(bb1, yf, g1));
     const float u2 = fmaf(a2, xf, fmaf(bb2, yf, g2));
@@ -56,8 +76,9 @@ __device__ __forceinline__ void bwd_pixels(const float4* pxa, const float2* pxb,
     s = (t > 0.0f) ? s : 0.0f;
     const float cs = cD * s;
     const float dt = t - D;
-    float gd2 = fmaf(b3 * cs, dt, cE * e);
-    float gtv = cs * fmaf(-b2, dt, 1.0f) * iv;
+    const float ex = (l == M) ? pxc[p] : 0.0f;   // the pair holding the maximum logit
+    float gd2 = fmaf(b3, ex, fmaf(b3 * cs, dt, cE * e));
+    float gtv = fmaf(-b2, ex, fmaf(cs, fmaf(-b2, dt, 1.0f), cF)) * iv;
     if (SAFE && (!(gd2 == gd2) || !(gtv == gtv))) { gd2 = 0.0f; gtv = 0.0f; }
     Gq0 = fmaf(gd2, v0, fmaf(gtv, u0, Gq0));
     Gq1 = fmaf(gd2, v1, fmaf(gtv, u1, Gq1));
@@ -74,11 +95,137 @@ __device__ __forceinline__ void bwd_pixels(const float4* pxa, const float2* pxb,
   }
 }
 
+// Blend-record form of bwd_pixels for one Gaussian (this lane); see the header comment.
+// pxb holds (cE, cD, b3 cD, 1e-20 cD) per pixel, pxc the argmax term cX.  Sums: GX = sum g_X p,
+// GY1/GY2 = sum c Y_i p.
+template <bool MASK, bool ARGMAX>
+__device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4* pxb, const float* pxc, int npx,
+                                                const float* k,
+                                                float wL, float cG, float qq, float b3, float b2, float kap,
+                                                float* GX, float* GY1, float* GY2, float& Gw, float& Gqq) {
+  const float kx0 = k[0], kx1 = k[1], kx2 = k[2], k10 = k[3], k11 = k[4], k12 = k[5], k20 = k[6], k21 = k[7],
+              k22 = k[8];
+  for (int p = 0; p < npx; ++p) {
+    const float4 pa = pxa[p];
+    const float4 pb = pxb[p];
+    const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y, cDb = pb.z, cF = pb.w;
+    const float X = fmaf(kx0, xf, fmaf(kx1, yf, kx2));
+    const float Y1 = fmaf(k10, xf, fmaf(k11, yf, k12));
+    const float Y2 = fmaf(k20, xf, fmaf(k21, yf, k22));
+    const float W = fmaf(Y1, Y1, Y2 * Y2);
+    const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
+    const float t = X * rr;
+    const float Wr = W * rr;
+    const float d2l = fmaf(-cG, Wr, wL);
+    const float e = __builtin_amdgcn_exp2f(d2l);
+    const float lt = fmaf(kap, t, d2l);
+    float s = __builtin_amdgcn_exp2f(fmaf(b3, lt, -M));
+    if (MASK) s = (t > 0.0f) ? s : 0.0f;
+    const float dt = t - D;
+    float g1 = fmaf(cD * s, fmaf(-b2, dt, 1.0f), cF);     // dL/dt
+    float gd2 = fmaf(cDb * s, dt, cE * e);                // dL/dd2
+    if (ARGMAX) {
+      const float ex = (b3 * lt == M) ? pxc[p] : 0.0f;   // the pair holding the maximum logit
+      g1 = fmaf(-b2, ex, g1);
+      gd2 = fmaf(b3, ex, gd2);
+    }
+    const float G = gd2 * qq;
+    const float h = fmaf(G, X, g1 + g1);
+    const float gX = fmaf(-(t * t), h, fmaf(G, t, g1 * rr));
+    const float c = h * (t * rr);
+    const float c1 = c * Y1, c2 = c * Y2;
+    GX[0] = fmaf(gX, xf, GX[0]); GX[1] = fmaf(gX, yf, GX[1]); GX[2] += gX;
+    GY1[0] = fmaf(c1, xf, GY1[0]); GY1[1] = fmaf(c1, yf, GY1[1]); GY1[2] += c1;
+    GY2[0] = fmaf(c2, xf, GY2[0]); GY2[1] = fmaf(c2, yf, GY2[1]); GY2[2] += c2;
+    Gw += gd2;
+    Gqq = fmaf(gd2, Wr, Gqq);
+  }
+}
+
+// Chain rule of the blend-record sums for Gaussian gid of view jb into the scene gradients
+// (10 atomics).  Float32 (float64 here doubles the kernel's registers and halves its
+// occupancy); against a float64 chain it differs by ~1e-6 of the largest gradient.
+__device__ __forceinline__ void rec2_chain(const Job& jb, const BwdJob& bj, int64_t gid, const float* GX,
+                                           const float* GY1, const float* GY2, float Gw, float Gqq) {
+  const float* P = jb.prec + 9 * gid;
+  const float* mu = jb.means + 3 * gid;
+  const float A[3][3] = {{P[0], 0.0f, 0.0f}, {P[1], P[4], 0.0f}, {P[2], P[5], P[8]}};
+  const float pv[3] = {mu[0] - jb.o[0], mu[1] - jb.o[1], mu[2] - jb.o[2]};
+  const float sc[3] = {1.0f / jb.fx, 1.0f / jb.fy, 1.0f};
+  float Mp[3][3], q[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      Mp[i][c] = fmaf(A[i][0], jb.R[3 * c], fmaf(A[i][1], jb.R[3 * c + 1], A[i][2] * jb.R[3 * c + 2])) * sc[c];
+    q[i] = fmaf(A[i][0], pv[0], fmaf(A[i][1], pv[1], A[i][2] * pv[2]));
+  }
+  const float qq = dot3(q, q);
+  const float iq = 1.0f / sqrtf(qq), iqq = 1.0f / qq;
+  const float n[3] = {q[0] * iq, q[1] * iq, q[2] * iq};
+  float e1[3], e2[3];
+  ortho_basis(n, e1, e2);
+  // dK[c][j] (c: coefficient of x', y', 1; j: X, Y1, Y2); the Y sums hold +c Y_i p
+  float dK[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    dK[c][0] = GX[c];
+    dK[c][1] = -GY1[c];
+    dK[c][2] = -GY2[c];
+  }
+  const float B[3][3] = {{q[0] * iqq, e1[0] * iq, e2[0] * iq},
+                         {q[1] * iqq, e1[1] * iq, e2[1] * iq},
+                         {q[2] * iqq, e1[2] * iq, e2[2] * iq}};
+  float dM[3][3], dB[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      dM[i][c] = fmaf(B[i][0], dK[c][0], fmaf(B[i][1], dK[c][1], B[i][2] * dK[c][2])) * sc[c];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) dB[i][j] = fmaf(Mp[i][0], dK[0][j], fmaf(Mp[i][1], dK[1][j], Mp[i][2] * dK[2][j]));
+  }
+  const float nb0 = fmaf(n[0], dB[0][0], fmaf(n[1], dB[1][0], n[2] * dB[2][0]));
+  const float nb1 = fmaf(n[0], dB[0][1], fmaf(n[1], dB[1][1], n[2] * dB[2][1]));
+  const float nb2 = fmaf(n[0], dB[0][2], fmaf(n[1], dB[1][2], n[2] * dB[2][2]));
+  const float eb = fmaf(e1[0], dB[0][1], fmaf(e1[1], dB[1][1], e1[2] * dB[2][1])) +
+                   fmaf(e2[0], dB[0][2], fmaf(e2[1], dB[1][2], e2[2] * dB[2][2]));
+  float dq[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+    dq[i] = fmaf(dB[i][0] - fmaf(n[i], fmaf(2.0f, nb0, eb), fmaf(e1[i], nb1, e2[i] * nb2)), iqq, -q[i] * Gqq);
+  // dA = dM R + dq pv^T (lower part); dmu = A^T dq; d(prec)[k][i] = dA[i][k]
+  if (bj.dmeans) {
+    float* dmu = bj.dmeans + 3 * gid;
+    atomicAdd(dmu + 0, fmaf(A[0][0], dq[0], fmaf(A[1][0], dq[1], A[2][0] * dq[2])));
+    atomicAdd(dmu + 1, fmaf(A[1][1], dq[1], A[2][1] * dq[2]));
+    atomicAdd(dmu + 2, A[2][2] * dq[2]);
+  }
+  if (bj.dprec) {
+    float* dp = bj.dprec + 9 * gid;
+    float dA[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int k = 0; k <= i; ++k)
+        dA[i][k] = fmaf(dM[i][0], jb.R[k], fmaf(dM[i][1], jb.R[3 + k], fmaf(dM[i][2], jb.R[6 + k], dq[i] * pv[k])));
+    atomicAdd(dp + 0, dA[0][0]);
+    atomicAdd(dp + 1, dA[1][0]);
+    atomicAdd(dp + 2, dA[2][0]);
+    atomicAdd(dp + 4, dA[1][1]);
+    atomicAdd(dp + 5, dA[2][1]);
+    atomicAdd(dp + 8, dA[2][2]);
+  }
+  if (bj.dw) atomicAdd(bj.dw + gid, Gw);
+}
+
 struct BwdArgs {
   const Job* jobs;
   const BwdJob* bjobs;
   int n_jobs;
   const float* rec;
+  const float* rec2;
+  const uint32_t* tile_np;
   const unsigned long long* off1;
   const unsigned long long* off2;
   const uint32_t* list;
@@ -103,7 +250,8 @@ template <int TS>
 __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   constexpr int NPIX = TS * TS;
   __shared__ float4 pxa[NPIX];   // x', y', M, D
-  __shared__ float2 pxb[NPIX];   // cE, cD
+  __shared__ float4 pxb[NPIX];   // cE, cD, b3 cD, 1e-20 cD
+  __shared__ float pxc[NPIX];    // cX (maximum-logit term)
 
   const int64_t gt = blockIdx.x;
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
@@ -124,12 +272,14 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   if (total == 0) return;
   const float ncnt = (float)total;
 
+  bool floor_px = false;
   for (int i = threadIdx.x; i < npx; i += 256) {
     const int px = tx * TS + (i % wv), py = ty * TS + (i / wv);
     const int64_t lp = (int64_t)py * jb.W + px;
     const int64_t pix = jb.pix_base + lp;
     const float M = A.st[pix];
     const float S = A.st[A.total_pix + pix];
+    const float Z = A.st[2 * A.total_pix + pix];
     const float E = A.st[3 * A.total_pix + pix];
     const float D = A.st[5 * A.total_pix + pix];
     const float gD = bj.gD ? bj.gD[lp] : 0.0f;
@@ -141,18 +291,28 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     if (!(cD == cD) || !(D == D)) cD = 0.0f;
     const float xr = jb.ray_xy ? jb.ray_xy[2 * lp] : (float)px - jb.cx;
     const float yr = jb.ray_xy ? jb.ray_xy[2 * lp + 1] : (float)py - jb.cy;
+    float cX = -cD * fmaf(-D, S, Z);
+    if (!(cX == cX) || fabsf(cX) > 3.0e38f) cX = 0.0f;
+    floor_px = floor_px || !(S >= 1e-11f * ncnt);
     pxa[i] = make_float4(xr, yr, M, D);
-    pxb[i] = make_float2(cE, cD);
+    pxb[i] = make_float4(cE, cD, jb.beta3 * cD, 1e-20f * cD);
+    pxc[i] = cX;
   }
-  __syncthreads();
+  const bool floor_tile = __syncthreads_or(floor_px ? 1 : 0) != 0;
 
   const float b3 = jb.beta3, b2 = jb.beta2;
   const float nb2l = -jb.beta2 * kLog2e;
   const float kd = -0.5f * kLog2e;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const bool safe = (A.job_flags[ji] & 1) != 0;   // the view has degenerate records
+  const int32_t jflags = A.job_flags[ji];
+  const bool safe = (jflags & kFlagDegenerate) != 0;   // the view has degenerate records
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
   const float ifx = 1.0f / jb.fx, ify = 1.0f / jb.fy;
+  const bool r2 = !(jflags & (kFlagDegenerate | kFlagNoBlendRec)) && jb.beta3 > 0.0f && jb.beta3 < 3.0e38f &&
+                  (jb.beta2 == jb.beta2) && fabsf(jb.beta2) < 3.0e38f;
+  const bool masked = A.dense || !A.tile_np || (A.tile_np[2 * gt] | A.tile_np[2 * gt + 1]) != 0;
+  const float kap = nb2l / b3;
+  const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
 
   for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 256) {
     const int64_t kk = c0 + lane;
@@ -160,6 +320,21 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     int64_t gid;
     if (A.dense) gid = kk;
     else gid = (kk < n1) ? (int64_t)A.list[s1 + kk] : (int64_t)A.list[s2 + (kk - n1)];
+    if (r2) {
+      const float4 q0 = rec24[gid * 4 + 0];
+      const float4 q1 = rec24[gid * 4 + 1];
+      const float4 q2 = rec24[gid * 4 + 2];
+      const float kk9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
+      float GX[3] = {0.f, 0.f, 0.f}, GY1[3] = {0.f, 0.f, 0.f}, GY2[3] = {0.f, 0.f, 0.f}, Gw = 0.f, Gqq = 0.f;
+      if (masked && floor_tile)
+        bwd_pixels_rec2<true, true>(pxa, pxb, pxc, npx, kk9, q2.y, q2.z, q2.w, b3, b2, kap, GX, GY1, GY2, Gw, Gqq);
+      else if (masked)
+        bwd_pixels_rec2<true, false>(pxa, pxb, pxc, npx, kk9, q2.y, q2.z, q2.w, b3, b2, kap, GX, GY1, GY2, Gw, Gqq);
+      else
+        bwd_pixels_rec2<false, false>(pxa, pxb, pxc, npx, kk9, q2.y, q2.z, q2.w, b3, b2, kap, GX, GY1, GY2, Gw, Gqq);
+      rec2_chain(jb, bj, gid, GX, GY1, GY2, Gw, Gqq);
+      continue;
+    }
     const float4 r0 = rec4[gid * 4 + 0];
     const float4 r1 = rec4[gid * 4 + 1];
     const float4 r2 = rec4[gid * 4 + 2];
@@ -167,10 +342,10 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     float Gq0 = 0.f, Gq1 = 0.f, Gq2 = 0.f, Ga0 = 0.f, Ga1 = 0.f, Ga2 = 0.f, Gb0 = 0.f, Gb1 = 0.f, Gb2 = 0.f;
     float Gg0 = 0.f, Gg1 = 0.f, Gg2 = 0.f, Gw = 0.f;
     if (safe) {
-      bwd_pixels<true>(pxa, pxb, npx, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w,
+      bwd_pixels<true>(pxa, pxb, pxc, npx, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w,
                        r3.y, b3, b2, nb2l, kd, Gq0, Gq1, Gq2, Ga0, Ga1, Ga2, Gb0, Gb1, Gb2, Gg0, Gg1, Gg2, Gw);
     } else {
-      bwd_pixels<false>(pxa, pxb, npx, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w,
+      bwd_pixels<false>(pxa, pxb, pxc, npx, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w,
                         r3.y, b3, b2, nb2l, kd, Gq0, Gq1, Gq2, Ga0, Ga1, Ga2, Gb0, Gb1, Gb2, Gg0, Gg1, Gg2, Gw);
     }
     // chain rule for this (tile, Gaussian) pair, straight into the scene gradients:
@@ -221,11 +396,11 @@ static hipError_t bwd_t(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
 }
 
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
-                            int n_jobs, const float* rec, const unsigned long long* off1,
-                            const unsigned long long* off2, const uint32_t* list, const float* st,
-                            int64_t total_pix, const int32_t* job_flags, int dense) {
-  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, off1, off2, list, nullptr, nullptr, st,
-            total_pix, job_flags, dense};
+                            int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
+                            const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
+                            const float* st, int64_t total_pix, const int32_t* job_flags, int dense) {
+  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, rec2, tile_np, off1, off2, list, nullptr,
+            nullptr, st, total_pix, job_flags, dense};
   switch (ts) {
     case 8: return bwd_t<8>(s, total_tiles, a);
     case 16: return bwd_t<16>(s, total_tiles, a);

@@ -46,7 +46,8 @@ struct Layout {
   int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0, reuse = 0;
   int64_t total_stiles = 0;
   int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
-  size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_mask = 0, o_count = 0,
+  size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_rec2 = 0, o_tnp = 0,
+         o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
          o_bjobs = 0, o_rect = 0, bytes = 0;
 };
@@ -193,6 +194,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->o_counters = p; p = align256(p + sizeof(Counters));
   L->o_flags = p; p = align256(p + sizeof(int32_t) * n_views);
   L->o_rec = p; p = align256(p + sizeof(float) * kRecFloats * (size_t)std::max<int64_t>(rec, 1));
+  L->o_rec2 = p; p = align256(p + sizeof(float) * kRec2Floats * (size_t)std::max<int64_t>(rec, 1));
   L->o_lmin = p; p = align256(p + sizeof(float) * tiles);
   L->o_ubm = p; p = align256(p + sizeof(float) * tiles);
   L->o_state = p; p = align256(p + sizeof(float) * 6 * pix);
@@ -200,6 +202,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     L->o_mask = p; p = align256(p + sizeof(unsigned long long) * std::max<int64_t>(maskw, 1));
     L->o_mask2 = p; p = align256(p + sizeof(unsigned long long) * std::max<int64_t>(maskw, 1));
     L->o_count = p; p = align256(p + sizeof(uint32_t) * tiles);
+    L->o_tnp = p; p = align256(p + sizeof(uint32_t) * 2 * tiles);
     L->o_off1 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
     L->o_off2 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
     L->o_partial = p; p = align256(p + sizeof(unsigned long long) * (scan_partial_count(tiles) + 1));
@@ -339,6 +342,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   Counters* cnt = at<Counters>(ws, L.o_counters);
   int32_t* flags = at<int32_t>(ws, L.o_flags);
   float* rec = at<float>(ws, L.o_rec);
+  float* rec2 = at<float>(ws, L.o_rec2);
   float* state = at<float>(ws, L.o_state);
   float* lmin = at<float>(ws, L.o_lmin);
   float* ubm = at<float>(ws, L.o_ubm);
@@ -346,12 +350,12 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   bool any_pose = false;
   for (int v = 0; v < n_views; ++v) any_pose = any_pose || views[v].pose_device != nullptr;
   if (any_pose) FE_HIP(launch_pose(s, at<Job>(ws, L.o_jobs), L.n_jobs));
-  FE_HIP(timed(s, FE_K_PREPROCESS, [&] { return launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, flags); }));
+  FE_HIP(timed(s, FE_K_PREPROCESS, [&] { return launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, rec2, flags); }));
   if (L.has_rays) FE_HIP(launch_ray_rects(s, dj, L.n_jobs, L.total_tiles, L.ts, flags));
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
-      return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, nullptr, nullptr, nullptr, state,
-                          L.total_pix, lmin, ubm, cnt, flags, L.save);
+      return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, rec2, nullptr, nullptr, nullptr,
+                          nullptr, state, L.total_pix, lmin, ubm, cnt, flags, L.save);
     }));
     return FE_OK;
   }
@@ -362,29 +366,30 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   unsigned long long* off2 = at<unsigned long long>(ws, L.o_off2);
   unsigned long long* partial = at<unsigned long long>(ws, L.o_partial);
   uint32_t* list = at<uint32_t>(ws, L.o_list);
+  uint32_t* tnp = at<uint32_t>(ws, L.o_tnp);
   // phase 1
   FE_HIP(timed(s, FE_K_MASK1, [&] {
-    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
-                            cnt);
+    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, tnp,
+                            flags, cnt);
   }));
   FE_HIP(timed(s, FE_K_SCAN1, [&] { return launch_scan(s, count, L.total_tiles, partial, off1, nullptr); }));
   FE_HIP(timed(s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }));
   FE_HIP(timed(s, FE_K_BLEND1, [&] {
-    return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state, L.total_pix,
-                        lmin, ubm, cnt, flags, L.save);
+    return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
+                        L.total_pix, lmin, ubm, cnt, flags, L.save);
   }));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
-    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, flags,
-                            cnt);
+    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, tnp,
+                            flags, cnt);
   }));
   FE_HIP(timed(s, FE_K_SCAN2, [&] {
     return launch_scan(s, count, L.total_tiles, partial, off2, off1 + L.total_tiles);
   }));
   FE_HIP(timed(s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }));
   FE_HIP(timed(s, FE_K_BLEND2, [&] {
-    return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, off1, off2, list, state, L.total_pix,
-                        lmin, ubm, cnt, flags, L.save);
+    return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
+                        L.total_pix, lmin, ubm, cnt, flags, L.save);
   }));
   return FE_OK;
 }
@@ -425,6 +430,7 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
   const bool dense = (L.mode == FE_MODE_DENSE);
   FE_HIP(timed(s, FE_K_BWD_BLEND, [&] {
     return launch_bwd_blend(s, L.ts, L.total_tiles, dj, dbj, L.n_jobs, at<float>(ws, L.o_rec),
+                            at<float>(ws, L.o_rec2), dense ? nullptr : at<uint32_t>(ws, L.o_tnp),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off1),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off2),
                             dense ? nullptr : at<uint32_t>(ws, L.o_list), at<float>(ws, L.o_state), L.total_pix,

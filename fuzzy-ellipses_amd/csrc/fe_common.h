@@ -12,6 +12,21 @@
 // With these, the whitened ray of pixel (x, y) is u = a*(x-cx) + b*(y-cy) + g, which equals
 // triu(prec)^T (r_cam @ R) of baseline/fm_render.py:33-52, and per pair
 //   t* = (q.u)/(u.u),  d0 = |t* u - q|^2,  d2 = w - d0/2   (fm_render.py:53-70).
+//
+// Blend record, 16 floats per (Gaussian, view), computed in float64 and rounded once:
+//   [0..2]  kX  = (a.q, b.q, g.q) / |q|^2
+//   [3..5]  kY1 = (a.e1, b.e1, g.e1) / |q|
+//   [6..8]  kY2 = (a.e2, b.e2, g.e2) / |q|
+//   [9]     w*log2(e)
+//   [10]    |q|^2 log2(e) / 2
+//   [11]    |q|^2
+//   [12..15] unused
+// where (q/|q|, e1, e2) is an orthonormal basis.  For p = (x-cx, y-cy, 1) the linear forms
+// X = kX.p and Y_i = kY_i.p are the components of the whitened ray u along q/|q|, e1, e2,
+// divided by |q|.  With W = Y1^2 + Y2^2 and den = X^2 + W:
+//   t* = (q.u)/(u.u) = X/den,   d0 = |q x u|^2/(u.u) = |q|^2 W/den,
+// so every term of t* and d0 is a sum of squares (no cancellation near the centre ray),
+// at 3 linear forms per pair instead of u, u.u, q.u and |t* u - q|^2.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,6 +37,29 @@ namespace fe {
 constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLn2 = 0.6931471805599453f;
 constexpr int kRecFloats = 16;
+constexpr int kRec2Floats = 16;
+// Smallest |q|^2 the blend record form accepts (smaller: the view uses the direct form).
+constexpr double kMinQQ = 1e-20;
+
+// Job flag bits (per view, set by the preprocess)
+constexpr int32_t kFlagDegenerate = 1;   // a non-finite input or a zero diagonal of U
+constexpr int32_t kFlagNoBlendRec = 2;   // some Gaussian has no valid blend record
+
+// Orthonormal vectors e1, e2 completing the unit vector n to a basis (Duff et al., "Building
+// an Orthonormal Basis, Revisited", JCGT 6(1), 2017).  Only W = Y1^2 + Y2^2 is used, which
+// does not depend on the choice of e1, e2 within the plane.
+template <typename T>
+__device__ __forceinline__ void ortho_basis(const T n[3], T e1[3], T e2[3]) {
+  const T s = n[2] >= T(0) ? T(1) : T(-1);
+  const T a = T(-1) / (s + n[2]);
+  const T b = n[0] * n[1] * a;
+  e1[0] = T(1) + s * n[0] * n[0] * a;
+  e1[1] = s * b;
+  e1[2] = -s * n[0];
+  e2[0] = b;
+  e2[1] = s + n[1] * n[1] * a;
+  e2[2] = -n[1];
+}
 
 // One view of one scene, as the kernels see it.  Built on the host by the planner.
 struct Job {
@@ -151,9 +189,11 @@ __device__ __forceinline__ float4 ray_rect(float xmin, float xmax, float ymin, f
 //   unbounded    : the tile cone is too wide for the bound (keep the pair)
 // sqrt(d0) = |q| sin(angle(q, line)) is |q|-Lipschitz in the ray angle; the angle from the
 // centre ray to any tile ray is at most the largest corner angle, bounded by its tangent.
+//   front        : like plus, with a 2^-10 relative margin on sd_hi < |q| (used only to let
+//                  the blend skip the t* > 0 test on a tile, never for membership)
 struct PairBound {
   float sd_lo, sd_hi, t_lb, t_any_lb, t_ub;
-  bool plus, unbounded;
+  bool plus, unbounded, front;
 };
 
 __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& tg) {
@@ -197,12 +237,14 @@ __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& t
   const float rho = sqrtf(fmaxf(dot3(ap, ap), dot3(am, am)));
   const float un = sqrtf(uu);
   pb.plus = (qu > 0.0f) && (pb.sd_hi < qn);
+  pb.front = (qu > 0.0f) && (pb.sd_hi * 1.0009765625f < qn);
   pb.t_lb = pb.plus ? sqrtf(fmaxf(fmaf(-pb.sd_hi, pb.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
   // |q.u|/|u| <= sqrt(qq - d0_lo) on the tile, |u| >= |u_c| - rho
   pb.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-pb.sd_lo, pb.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
   // upper bound of t* on the tile: q.u/|u| <= min(|q|, q.u_c/|u_c| + |q| tan(angle)), |u| >= |u_c| - rho
   pb.t_ub = (un > rho) ? fminf(qn, qu / un + dth) / (un - rho) : INFINITY;
   if (!(pb.sd_lo == pb.sd_lo)) pb.unbounded = true;  // NaN anywhere -> keep
+  pb.front = pb.front && !pb.unbounded;
   return pb;
 }
 

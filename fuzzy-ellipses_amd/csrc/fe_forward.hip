@@ -72,6 +72,7 @@ __global__ void k_pose(Job* jobs, int n_jobs) {
 // ------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs,
                                                     float* __restrict__ rec,
+                                                    float* __restrict__ rec2,
                                                     int32_t* __restrict__ job_flags) {
   const Job& jb = jobs[blockIdx.y];
   const int j = blockIdx.x * 256 + threadIdx.x;
@@ -108,10 +109,60 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
   for (int i = 0; i < 15; ++i) finite = finite && isfinite(out[i]);
   const bool degenerate = !finite || u00 == 0.0f || u11 == 0.0f || u22 == 0.0f;
   out[15] = __int_as_float(degenerate ? 1 : 0);
-  if (degenerate) atomicOr(job_flags + blockIdx.y, 1);
+  if (degenerate) atomicOr(job_flags + blockIdx.y, kFlagDegenerate);
   float4* dst = reinterpret_cast<float4*>(rec + (jb.rec_base + j) * kRecFloats);
 #pragma unroll
   for (int i = 0; i < 4; ++i) dst[i] = make_float4(out[4 * i], out[4 * i + 1], out[4 * i + 2], out[4 * i + 3]);
+
+  // blend record (fe_common.h), in float64 from the same float inputs
+  const double A0[3] = {(double)u00, 0.0, 0.0};
+  const double A1[3] = {(double)u01, (double)u11, 0.0};
+  const double A2[3] = {(double)u02, (double)u12, (double)u22};
+  const double* Ar[3] = {A0, A1, A2};
+  const double pd[3] = {(double)mu[0] - (double)jb.o[0], (double)mu[1] - (double)jb.o[1],
+                        (double)mu[2] - (double)jb.o[2]};
+  const double sc[3] = {1.0 / (double)jb.fx, 1.0 / (double)jb.fy, 1.0};
+  double Md[3][3], qd[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      Md[i][k] = (Ar[i][0] * (double)jb.R[3 * k] + Ar[i][1] * (double)jb.R[3 * k + 1] +
+                  Ar[i][2] * (double)jb.R[3 * k + 2]) * sc[k];
+    qd[i] = Ar[i][0] * pd[0] + Ar[i][1] * pd[1] + Ar[i][2] * pd[2];
+  }
+  const double qq = qd[0] * qd[0] + qd[1] * qd[1] + qd[2] * qd[2];
+  float o2[16];
+  bool ok = !degenerate && qq >= kMinQQ && qq < 1e30;
+  if (ok) {
+    const double iq = 1.0 / sqrt(qq);
+    const double n[3] = {qd[0] * iq, qd[1] * iq, qd[2] * iq};
+    double e1[3], e2[3];
+    ortho_basis(n, e1, e2);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double mq = Md[0][c] * qd[0] + Md[1][c] * qd[1] + Md[2][c] * qd[2];
+      const double m1 = Md[0][c] * e1[0] + Md[1][c] * e1[1] + Md[2][c] * e1[2];
+      const double m2 = Md[0][c] * e2[0] + Md[1][c] * e2[1] + Md[2][c] * e2[2];
+      o2[c] = (float)(mq / qq);
+      o2[3 + c] = (float)(m1 * iq);
+      o2[6 + c] = (float)(m2 * iq);
+    }
+    o2[9] = (float)((double)w * 1.4426950408889634);
+    o2[10] = (float)(qq * 0.7213475204444817);
+    o2[11] = (float)qq;
+#pragma unroll
+    for (int i = 0; i < 12; ++i) ok = ok && isfinite(o2[i]);
+  }
+  if (!ok) {
+#pragma unroll
+    for (int i = 0; i < 12; ++i) o2[i] = 0.0f;
+    if (!degenerate) atomicOr(job_flags + blockIdx.y, kFlagNoBlendRec);
+  }
+  o2[12] = o2[13] = o2[14] = o2[15] = 0.0f;
+  float4* dst2 = reinterpret_cast<float4*>(rec2 + (jb.rec_base + j) * kRec2Floats);
+#pragma unroll
+  for (int i = 0; i < 4; ++i) dst2[i] = make_float4(o2[4 * i], o2[4 * i + 1], o2[4 * i + 2], o2[4 * i + 3]);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -139,10 +190,12 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
                                                    uint32_t* __restrict__ count,
                                                    const float* __restrict__ tile_lmin,
                                                    float* __restrict__ tile_ubm,
+                                                   uint32_t* __restrict__ tile_np,
                                                    const int32_t* __restrict__ job_flags,
                                                    Counters* __restrict__ cnt_out) {
   constexpr int NS = kMaxSF * kMaxSF;
   __shared__ uint32_t sh_cnt[4][NS];
+  __shared__ uint32_t sh_np[4][NS];
   __shared__ float sh_ubm[4][NS];
   const int64_t gs = blockIdx.x;
   const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
@@ -215,6 +268,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
   __syncthreads();
   const float tau_s = (PHASE == 2) ? tau_grp : 0.0f;
   uint32_t cnt[NS];
+  uint32_t npbits = 0;   // bit k: sub-tile k lists a pair not known to be in front of the camera
   float ubm[NS];
 #pragma unroll
   for (int k = 0; k < NS; ++k) { cnt[k] = 0; ubm[k] = -INFINITY; }
@@ -256,7 +310,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
         const int64_t row = jb.mask_base + tile * jb.nw;
         const unsigned long long old = (PHASE == 2) ? mask1[row + c] : 0ull;
         const bool listed = (PHASE == 2) && ((old >> lane) & 1ull);
-        bool keep = false;
+        bool keep = false, front = false;
         if (PHASE == 1) {
           keep = forced;
         }
@@ -264,6 +318,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
           if (keep_s) {
             const PairBound pb = pair_bound(r, tg[k]);
             const float w = r[12];
+            front = pb.front;
             if (PHASE == 1) {
               keep = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
             } else {
@@ -282,6 +337,8 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
           else mask2[row + c] = word;
         }
         cnt[k] += (uint32_t)__popcll(word);
+        // listed pairs not provably in front of the camera on the whole tile
+        if (__ballot(keep && !front) != 0ull) npbits |= 1u << k;
       }
     }
   }
@@ -292,7 +349,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
       for (int o = 32; o > 0; o >>= 1) u = fmaxf(u, __shfl_xor(u, o, 64));
       ubm[k] = u;
     }
-    if (lane == 0) { sh_cnt[wave][k] = cnt[k]; sh_ubm[wave][k] = ubm[k]; }
+    if (lane == 0) { sh_cnt[wave][k] = cnt[k]; sh_np[wave][k] = (npbits >> k) & 1u; sh_ubm[wave][k] = ubm[k]; }
   }
   __syncthreads();
   // one wave finishes the sub-tiles: lane k owns sub-tile k
@@ -301,6 +358,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
       const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
       const int64_t gt = jb.tile_base + tile;
       uint32_t total = sh_cnt[0][k] + sh_cnt[1][k] + sh_cnt[2][k] + sh_cnt[3][k];
+      uint32_t np = sh_np[0][k] | sh_np[1][k] | sh_np[2][k] | sh_np[3][k];
       if (PHASE == 2) {
         const float ubm_t = fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k]));
         const bool dense_tile = (ubm_t - lmin[k] >= jb.floor_thr);   // false when nothing in doubt
@@ -316,11 +374,15 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
             c2 += (uint32_t)__popcll(word);
           }
           total = c2;
+          np = 1;   // completed without bounds: none is known to be in front
           if (lane == 0) atomicAdd(&cnt_out->floor_tiles, 1ull);
         }
         if (lane == 0) tile_ubm[gt] = dense_tile ? -INFINITY : ubm_t;
       }
-      if (lane == 0) count[gt] = total;
+      if (lane == 0) {
+        count[gt] = total;
+        tile_np[2 * gt + (PHASE - 1)] = np;
+      }
     }
   }
 }
@@ -533,10 +595,70 @@ __device__ __forceinline__ void blend_batch(const float4* lds, int nbr, float xf
   }
 }
 
+// The same batch through blend records (fe_common.h): per pair 3 linear forms, one
+// reciprocal and two exponentials.  Logits are kept as lt = l/b3 (log2 units, b3 > 0), so
+// the running maximum Mt is of lt and the reference M = b3*Mt is what S and Z are relative to
+// (the value saved for the second pass and the backward).  MASK applies the t* > 0 test; a
+// tile whose listed pairs are all in front of the camera skips it.
+template <int SUB, bool MASK>
+__device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, float xf, float yf, float b3, float kap,
+                                                 float& M, float& Mt, float& S, float& Z, float& E, float& Zall) {
+  for (int k0 = 0; k0 < nbr; k0 += SUB) {
+    float ls[SUB], tsv[SUB];
+    float mx = Mt;
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      const float4 r0 = lds[(k0 + s) * 3 + 0];
+      const float4 r1 = lds[(k0 + s) * 3 + 1];
+      const float4 r2 = lds[(k0 + s) * 3 + 2];
+      // kX = r0.xyz, kY1 = (r0.w, r1.x, r1.y), kY2 = (r1.z, r1.w, r2.x), wL = r2.y, cG = r2.z
+      const float X = fmaf(r0.x, xf, fmaf(r0.y, yf, r0.z));
+      const float Y1 = fmaf(r0.w, xf, fmaf(r1.x, yf, r1.y));
+      const float Y2 = fmaf(r1.z, xf, fmaf(r1.w, yf, r2.x));
+      const float W = fmaf(Y1, Y1, Y2 * Y2);
+      const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
+      const float t = X * rr;
+      // log2(e) * d2, rounded exactly as the backward recomputes it (it finds the pair
+      // holding the maximum logit by equality)
+      const float d2l = fmaf(-r2.z, W * rr, r2.y);
+      E += __builtin_amdgcn_exp2f(d2l);
+      const float lt = fmaf(kap, t, d2l);               // log2(e) * (b3 d2 - b2 t) / b3
+      Zall += t;
+      ls[s] = lt;
+      tsv[s] = t;
+      mx = fmaxf(mx, lt);
+    }
+    if (mx > Mt) {
+      const float Mn = b3 * mx;
+      const float sc = __builtin_amdgcn_exp2f(M - Mn);
+      S *= sc;
+      Z *= sc;
+      M = Mn;
+      Mt = mx;
+    }
+#pragma unroll
+    for (int s = 0; s < SUB; ++s) {
+      float sv = __builtin_amdgcn_exp2f(fmaf(b3, ls[s], -M));
+      if (MASK) sv = (tsv[s] > 0.0f) ? sv : 0.0f;
+      S += sv;
+      Z = fmaf(sv, tsv[s], Z);
+    }
+  }
+}
+
+// Whether a view's blends use the blend records: no degenerate record, every record valid,
+// and b3 > 0 (the running maximum is taken over l/b3).
+__device__ __forceinline__ bool use_rec2(const Job& jb, int32_t flags) {
+  return !(flags & (kFlagDegenerate | kFlagNoBlendRec)) && jb.beta3 > 0.0f && jb.beta3 < 3.0e38f &&
+         (jb.beta2 == jb.beta2) && fabsf(jb.beta2) < 3.0e38f;
+}
+
 struct BlendArgs {
   const Job* jobs;
   int n_jobs;
   const float* rec;
+  const float* rec2;
+  const uint32_t* tile_np;    // per tile and phase: 1 if a listed pair is not known to be in front
   const unsigned long long* off1;
   const unsigned long long* off2;
   const uint32_t* list;
@@ -577,7 +699,11 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   const float b3 = jb.beta3;
   const float nb2l = -jb.beta2 * kLog2e;
   const float kd = -0.5f * kLog2e;
-  const bool safe = (A.job_flags[ji] & 1) != 0;   // the view has degenerate records
+  const int32_t jflags = A.job_flags[ji];
+  const bool safe = (jflags & kFlagDegenerate) != 0;   // the view has degenerate records
+  const bool r2 = use_rec2(jb, jflags);
+  const float kap = r2 ? nb2l / b3 : 0.0f;
+  const bool masked = (PHASE == 0) || !A.tile_np || A.tile_np[2 * gt + (PHASE == 2 ? 1 : 0)] != 0;
 
   float M, S, Z, E, Zall;
   if (PHASE == 2) {
@@ -591,16 +717,39 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   }
   // a NaN logit seen by the phase-1 pass (kept in the sixth state plane between the passes)
   bool nanf = (PHASE == 2) ? (A.st[5 * A.total_pix + pix] != 0.0f) : false;
+  float Mt = (r2 && PHASE == 2) ? M / b3 : -3.402823466e38f;
 
   int64_t start, end;
   if (PHASE == 0) { start = 0; end = jb.n; }
   else if (PHASE == 1) { start = (int64_t)A.off1[gt]; end = (int64_t)A.off1[gt + 1]; }
   else { start = (int64_t)A.off2[gt]; end = (int64_t)A.off2[gt + 1]; }
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
+  const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
 
   for (int64_t base = start; base < end; base += GB) {
     const int nb = (int)min<int64_t>(GB, end - base);
     const int nbr = (nb + SUB - 1) / SUB * SUB;
+    if (r2) {
+      // 3 float4 per record; null record: X = 0, Y1 = 1, wL = -inf -> contributes nothing
+      for (int i = threadIdx.x; i < nbr * 3; i += NT) {
+        const int k = i / 3, part = i - 3 * k;
+        float4 v;
+        if (k < nb) {
+          const int64_t gid = (PHASE == 0) ? base + k : (int64_t)A.list[base + k];
+          v = rec24[gid * 4 + part];
+        } else {
+          v = (part == 1) ? make_float4(0.f, 1.f, 0.f, 0.f)
+              : (part == 2) ? make_float4(0.f, -INFINITY, 0.f, 0.f)
+                            : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+        lds[i] = v;
+      }
+      __syncthreads();
+      if (masked) blend_batch_rec2<SUB, true>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+      else blend_batch_rec2<SUB, false>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+      __syncthreads();
+      continue;
+    }
     for (int i = threadIdx.x; i < nbr * 4; i += NT) {
       const int k = i >> 2, part = i & 3;
       float4 v;
@@ -671,8 +820,9 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
     // behind the camera pushes the true max logit far above M
     const float ubm = A.tile_ubm[gt];
     const float lhs = -43.185f + __log2f((float)max(jb.n, 1)) + fmaxf(0.0f, ubm - M);
-    // (a view with degenerate records lists every pair, so nothing was dropped there)
-    const bool fl = inimg && !safe && !(lhs < ((S > 0.0f) ? __log2f(S) : -INFINITY));
+    // (a view with degenerate records lists every pair, and so does a tile completed densely:
+    // nothing was dropped there and the floor term is exact)
+    const bool fl = inimg && !safe && ncnt < (float)jb.n && !(lhs < ((S > 0.0f) ? __log2f(S) : -INFINITY));
     const int any = __syncthreads_or(fl ? 1 : 0);
     if (threadIdx.x == 0) {
       if (any) atomicAdd(&A.cnt->floor_unresolved, 1ull);
@@ -716,24 +866,24 @@ hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t 
   return hipGetLastError();
 }
 
-hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec,
+hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec, float* rec2,
                              int32_t* job_flags) {
   if (max_n <= 0) return hipSuccess;
-  hipLaunchKernelGGL(k_preprocess, dim3((max_n + 255) / 256, n_jobs), dim3(256), 0, s, jobs, rec,
+  hipLaunchKernelGGL(k_preprocess, dim3((max_n + 255) / 256, n_jobs), dim3(256), 0, s, jobs, rec, rec2,
                      job_flags);
   return hipGetLastError();
 }
 
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
                             int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
-                            uint32_t* count, const float* tile_lmin, float* tile_ubm, const int32_t* job_flags,
-                            Counters* cnt) {
+                            uint32_t* count, const float* tile_lmin, float* tile_ubm, uint32_t* tile_np,
+                            const int32_t* job_flags, Counters* cnt) {
   if (phase == 1)
     hipLaunchKernelGGL(k_tile_mask<1>, dim3((unsigned)total_stiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
-                       mask2, count, tile_lmin, tile_ubm, job_flags, cnt);
+                       mask2, count, tile_lmin, tile_ubm, tile_np, job_flags, cnt);
   else
     hipLaunchKernelGGL(k_tile_mask<2>, dim3((unsigned)total_stiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
-                       mask2, count, tile_lmin, tile_ubm, job_flags, cnt);
+                       mask2, count, tile_lmin, tile_ubm, tile_np, job_flags, cnt);
   return hipGetLastError();
 }
 
@@ -777,11 +927,13 @@ static hipError_t launch_blend_t(hipStream_t s, int gb, int phase, int64_t total
 }
 
 hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_tiles, const Job* jobs,
-                        int n_jobs, const float* rec, const unsigned long long* off1,
+                        int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
+                        const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
                         float* tile_lmin, const float* tile_ubm, Counters* cnt, const int32_t* job_flags,
                         int save) {
-  BlendArgs a{jobs, n_jobs, rec, off1, off2, list, st, total_pix, tile_lmin, tile_ubm, cnt, job_flags, save};
+  BlendArgs a{jobs, n_jobs, rec, rec2, tile_np, off1, off2, list, st, total_pix, tile_lmin, tile_ubm, cnt, job_flags,
+              save};
   switch (ts) {
     case 8: return launch_blend_t<8>(s, gb, phase, total_tiles, a);
     case 16: return launch_blend_t<16>(s, gb, phase, total_tiles, a);

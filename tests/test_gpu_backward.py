@@ -53,3 +53,45 @@ def test_backward_two_views_1024(fea, scenes_mod):
     errs = [_rel(g, r) for g, r in zip(got, ref)]
     print(f"backward N=1024 2 views: rel err means {errs[0]:.2e} prec {errs[1]:.2e} w {errs[2]:.2e}")
     assert max(errs) <= GRAD_RTOL
+
+
+@pytest.mark.parametrize("mode", ["dense", "culled"])
+def test_backward_camera_inside_cube(fea, scenes_mod, mode):
+    """200 Gaussians in the unit cube seen from inside it: about 30% of the pairs lie behind
+    the camera, so tiles take the masked blend; the 1e-20 floor stays invisible (S >= 1e-6)."""
+    sc = scenes_mod
+    rng = np.random.default_rng(7)
+    means, prec, w = sc.random_gaussians(rng, 200)
+    cam = sc.make_camera(np.array([0.12, -0.07, 0.18]), 64, 64, 40.0)
+    scene = sc.Scene(means, prec, w, [cam])
+    rng = np.random.default_rng(33)
+    gd = rng.normal(size=cam.H * cam.W)
+    ga = rng.normal(size=cam.H * cam.W) * 10
+    ref = go.grads(scene, [cam], [gd], [ga])
+    got = _gpu_grads(fea, scene, [cam], [gd], [ga], mode)
+    errs = [_rel(g, r) for g, r in zip(got, ref)]
+    print(f"backward camera inside {mode}: rel err means {errs[0]:.2e} prec {errs[1]:.2e} w {errs[2]:.2e}")
+    assert max(errs) <= GRAD_RTOL
+
+
+@pytest.mark.parametrize("mode", ["dense", "culled"])
+@pytest.mark.parametrize("centre", [(0.02, -0.03, 0.05), (0.3, -0.2, 0.4)])
+def test_backward_weight_floor(fea, scenes_mod, mode, centre):
+    """1024 Gaussians on the unit shell seen from near its centre: the pairs behind the camera
+    hold the maximum logit and the 1e-20 weight floor is visible at every pixel (near the
+    centre it dominates; off centre it is comparable to the listed weights).  Autograd of the
+    reference then has two more terms: dD/dt = 1e-20/(S + 1e-20 n) for every pair, and the
+    gradient through the maximum logit of jax_stable_exp."""
+    sc = scenes_mod
+    rng = np.random.default_rng(8)
+    means, prec, w = sc.random_gaussians(rng, 1024, shell=True)
+    cam = sc.make_camera(np.array(centre), 64, 64, 40.0)
+    scene = sc.Scene(means, prec, w, [cam])
+    rng = np.random.default_rng(34)
+    gd = rng.normal(size=cam.H * cam.W)
+    ga = rng.normal(size=cam.H * cam.W) * 10
+    ref = go.grads(scene, [cam], [gd], [ga])
+    got = _gpu_grads(fea, scene, [cam], [gd], [ga], mode)
+    errs = [_rel(g, r) for g, r in zip(got, ref)]
+    print(f"backward weight floor {centre} {mode}: rel err means {errs[0]:.2e} prec {errs[1]:.2e} w {errs[2]:.2e}")
+    assert max(errs) <= GRAD_RTOL

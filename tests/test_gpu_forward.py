@@ -117,3 +117,43 @@ def test_degenerate_inputs_follow_oracle(fea, scenes_mod, mode, kind):
     assert ea <= ALPHA_TOL and ed <= DEPTH_RTOL
     if mode == "culled":
         assert r.stats()["floor_unresolved"] == 0
+
+
+def _inside_scene(sc, kind):
+    """Cameras inside the Gaussian cloud: pairs behind the camera (t* <= 0, masked from the
+    depth softmax but counted in alpha and in the max logit).
+      cube  : 200 Gaussians in the unit cube; the max logit is a masked pair at almost every
+              pixel, the 1e-20 floor stays invisible
+      centre: 1024 on the unit shell, camera near the centre; the floor dominates every pixel
+      offc  : the same shell, camera off centre; floor and listed weights are comparable"""
+    if kind == "cube":
+        rng = np.random.default_rng(7)
+        means, prec, w = sc.random_gaussians(rng, 200)
+        centre = (0.12, -0.07, 0.18)
+    else:
+        rng = np.random.default_rng(8)
+        means, prec, w = sc.random_gaussians(rng, 1024, shell=True)
+        centre = (0.02, -0.03, 0.05) if kind == "centre" else (0.3, -0.2, 0.4)
+    cam = sc.make_camera(np.array(centre), 64, 64, 40.0)
+    return sc.Scene(means, prec, w, [cam])
+
+
+@pytest.mark.parametrize("mode", ["dense", "culled"])
+@pytest.mark.parametrize("kind", ["cube", "centre", "offc"])
+def test_camera_inside_cloud(fea, scenes_mod, mode, kind):
+    import torch
+    sc = scenes_mod
+    scene = _inside_scene(sc, kind)
+    cam = scene.cameras[0]
+    r = fea.Renderer(tile=16, batch=64, mode=mode)
+    st = fea.SceneTensors.from_scene(scene)
+    (d, a), = r.forward([st], [(0, cam, scene.beta2, scene.beta3)])
+    torch.cuda.synchronize()
+    od, oa = _oracle_view(sc, scene, cam)
+    ed, ea = _errors(d.cpu().numpy().ravel(), a.cpu().numpy().ravel(), od, oa)
+    stats = r.stats()
+    print(f"camera inside ({kind}) {mode}: max depth rel err {ed:.3e}, max alpha abs err {ea:.3e}, "
+          f"floor tiles {stats.get('floor_tiles')}")
+    assert ea <= ALPHA_TOL and ed <= DEPTH_RTOL
+    if mode == "culled":
+        assert stats["floor_unresolved"] == 0
