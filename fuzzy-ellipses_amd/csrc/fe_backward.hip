@@ -27,8 +27,8 @@
 // X, Y1, Y2 of p = (x', y', 1) instead (t = X r, d0 = |q|^2 W r, r = 1/(X^2 + W)):
 //   g_X  = g_t r + G t - t^2 h,   g_Yi = -c Y_i,
 //   with G = g_d2 |q|^2, h = 2 g_t + G X, c = h t r,
-// accumulating 11 sums per (tile, Gaussian): sum g_X p (3), sum c Y_i p (3 + 3), sum g_d2,
-// and sum g_d2 W r (d2 depends on |q|^2 directly: dd2/d|q|^2 = -W r / 2).  The chain rule
+// with 11 sums per (Gaussian, view): sum g_X p (3), sum c Y_i p (3 + 3), sum g_d2, and
+// sum g_d2 W r (d2 depends on |q|^2 directly: dd2/d|q|^2 = -W r / 2).  The chain rule
 // to (mu, U) goes through the blend record K = M'^T B, B = [q/|q|^2, e1/|q|, e2/|q|]
 // (M' = [a b g]): dM' = B dK^T, dB = M' dK, and for the basis the derivative that keeps
 // e1, e2 orthonormal with no rotation in their plane (W does not depend on that rotation):
@@ -92,53 +92,6 @@ __device__ __forceinline__ void bwd_pixels(const float4* pxa, const float4* pxb,
     Gb0 = fmaf(gu0, yf, Gb0); Gb1 = fmaf(gu1, yf, Gb1); Gb2 = fmaf(gu2, yf, Gb2);
     Gg0 += gu0; Gg1 += gu1; Gg2 += gu2;
     Gw += gd2;
-  }
-}
-
-// Blend-record form of bwd_pixels for one Gaussian (this lane); see the header comment.
-// pxb holds (cE, cD, b3 cD, 1e-20 cD) per pixel, pxc the argmax term cX.  Sums: GX = sum g_X p,
-// GY1/GY2 = sum c Y_i p.
-template <bool MASK, bool ARGMAX>
-__device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4* pxb, const float* pxc, int npx,
-                                                const float* k,
-                                                float wL, float cG, float qq, float b3, float b2, float kap,
-                                                float* GX, float* GY1, float* GY2, float& Gw, float& Gqq) {
-  const float kx0 = k[0], kx1 = k[1], kx2 = k[2], k10 = k[3], k11 = k[4], k12 = k[5], k20 = k[6], k21 = k[7],
-              k22 = k[8];
-  for (int p = 0; p < npx; ++p) {
-    const float4 pa = pxa[p];
-    const float4 pb = pxb[p];
-    const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y, cDb = pb.z, cF = pb.w;
-    const float X = fmaf(kx0, xf, fmaf(kx1, yf, kx2));
-    const float Y1 = fmaf(k10, xf, fmaf(k11, yf, k12));
-    const float Y2 = fmaf(k20, xf, fmaf(k21, yf, k22));
-    const float W = fmaf(Y1, Y1, Y2 * Y2);
-    const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
-    const float t = X * rr;
-    const float Wr = W * rr;
-    const float d2l = fmaf(-cG, Wr, wL);
-    const float e = __builtin_amdgcn_exp2f(d2l);
-    const float lt = fmaf(kap, t, d2l);
-    float s = __builtin_amdgcn_exp2f(fmaf(b3, lt, -M));
-    if (MASK) s = (t > 0.0f) ? s : 0.0f;
-    const float dt = t - D;
-    float g1 = fmaf(cD * s, fmaf(-b2, dt, 1.0f), cF);     // dL/dt
-    float gd2 = fmaf(cDb * s, dt, cE * e);                // dL/dd2
-    if (ARGMAX) {
-      const float ex = (b3 * lt == M) ? pxc[p] : 0.0f;   // the pair holding the maximum logit
-      g1 = fmaf(-b2, ex, g1);
-      gd2 = fmaf(b3, ex, gd2);
-    }
-    const float G = gd2 * qq;
-    const float h = fmaf(G, X, g1 + g1);
-    const float gX = fmaf(-(t * t), h, fmaf(G, t, g1 * rr));
-    const float c = h * (t * rr);
-    const float c1 = c * Y1, c2 = c * Y2;
-    GX[0] = fmaf(gX, xf, GX[0]); GX[1] = fmaf(gX, yf, GX[1]); GX[2] += gX;
-    GY1[0] = fmaf(c1, xf, GY1[0]); GY1[1] = fmaf(c1, yf, GY1[1]); GY1[2] += c1;
-    GY2[0] = fmaf(c2, xf, GY2[0]); GY2[1] = fmaf(c2, yf, GY2[1]); GY2[2] += c2;
-    Gw += gd2;
-    Gqq = fmaf(gd2, Wr, Gqq);
   }
 }
 
@@ -219,6 +172,7 @@ __device__ __forceinline__ void rec2_chain(const Job& jb, const BwdJob& bj, int6
   if (bj.dw) atomicAdd(bj.dw + gid, Gw);
 }
 
+
 struct BwdArgs {
   const Job* jobs;
   const BwdJob* bjobs;
@@ -229,29 +183,115 @@ struct BwdArgs {
   const unsigned long long* off1;
   const unsigned long long* off2;
   const uint32_t* list;
-  const unsigned long long* mask1;
-  const unsigned long long* mask2;
   const float* st;
   int64_t total_pix;
   const int32_t* job_flags;
+  float* gsum;          // the 11 blend-record sums per (Gaussian, view) record, one plane each
+  int64_t total_rec;    // plane stride
   int dense;
 };
 
-// One block (4 waves) per tile.  Lane = one Gaussian of the tile's lists (phase 1 then
-// phase 2, or all Gaussians in dense mode); waves take chunks of 64 Gaussians round-robin.
-// Each lane loops over the tile's pixels, whose per-pixel terms (x', y', M, D, cE, cD) are
-// staged once in LDS and read as wave-uniform broadcasts, and keeps its Gaussian's 13 sums
-// in registers -- no cross-lane reduction.  The chain rule then maps the sums to the 10
-// gradient components of (mu, upper U, w), added to the scene gradients with one
-// device-scope atomic each.  (Grouping several tiles per block to share those atomics was
-// measured slower: lanes of one wave then cover different numbers of tiles, or waves wait at
-// barriers.)
+// Sums per record.  Planes, not records of 11 floats: device-scope float atomics to one
+// cache line serialise, and a record's 11 sums are added together by every tile.
+constexpr int kGsumPlanes = 11;
+
+// Per-pixel terms of the backward (upstream gradients through the saved aggregates).
+struct PixTerms {
+  float xf, yf, M, D, cE, cD, cDb, cF, cX;
+  bool floor_px;
+};
+
+__device__ __forceinline__ PixTerms pix_terms(const BwdArgs& A, const Job& jb, const BwdJob& bj, int64_t lp, int px,
+                                              int py, float ncnt) {
+  PixTerms t;
+  t.xf = jb.ray_xy ? jb.ray_xy[2 * lp] : (float)px - jb.cx;
+  t.yf = jb.ray_xy ? jb.ray_xy[2 * lp + 1] : (float)py - jb.cy;
+  const int64_t pix = jb.pix_base + lp;
+  const float M = A.st[pix];
+  const float S = A.st[A.total_pix + pix];
+  const float Z = A.st[2 * A.total_pix + pix];
+  const float E = A.st[3 * A.total_pix + pix];
+  const float D = A.st[5 * A.total_pix + pix];
+  const float gD = bj.gD ? bj.gD[lp] : 0.0f;
+  const float gA = bj.gA ? bj.gA[lp] : 0.0f;
+  const float den = S + 1e-20f * ncnt;
+  float cE = gA * expf(-E);
+  float cD = (den > 0.0f) ? gD / den : 0.0f;
+  if (!(cE == cE)) cE = 0.0f;
+  if (!(cD == cD) || !(D == D)) cD = 0.0f;
+  float cX = -cD * fmaf(-D, S, Z);
+  if (!(cX == cX) || fabsf(cX) > 3.0e38f) cX = 0.0f;
+  t.M = M; t.D = D; t.cE = cE; t.cD = cD; t.cDb = jb.beta3 * cD; t.cF = 1e-20f * cD; t.cX = cX;
+  t.floor_px = !(S >= 1e-11f * ncnt);
+  return t;
+}
+
+// Blend-record loop over pixels [p0, p1) for one Gaussian (this lane); see the header
+// comment.  pxb holds (cE, cD, b3 cD, 1e-20 cD), pxc the maximum-logit term cX.
+// Sums: G[0..2] = sum g_X p, G[3..5] = sum c Y1 p, G[6..8] = sum c Y2 p, G[9] = sum g_d2,
+// G[10] = sum g_d2 W r.
+template <bool MASK, bool ARGMAX>
+__device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4* pxb, const float* pxc, int p0,
+                                                int p1, const float* k, float wL, float cG, float qq, float b3,
+                                                float b2, float kap, float* G) {
+  const float kx0 = k[0], kx1 = k[1], kx2 = k[2], k10 = k[3], k11 = k[4], k12 = k[5], k20 = k[6], k21 = k[7],
+              k22 = k[8];
+  for (int p = p0; p < p1; ++p) {
+    const float4 pa = pxa[p];
+    const float4 pb = pxb[p];
+    const float xf = pa.x, yf = pa.y, M = pa.z, D = pa.w, cE = pb.x, cD = pb.y, cDb = pb.z, cF = pb.w;
+    const float X = fmaf(kx0, xf, fmaf(kx1, yf, kx2));
+    const float Y1 = fmaf(k10, xf, fmaf(k11, yf, k12));
+    const float Y2 = fmaf(k20, xf, fmaf(k21, yf, k22));
+    const float W = fmaf(Y1, Y1, Y2 * Y2);
+    const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
+    const float t = X * rr;
+    const float Wr = W * rr;
+    const float d2l = fmaf(-cG, Wr, wL);
+    const float e = __builtin_amdgcn_exp2f(d2l);
+    const float lt = fmaf(kap, t, d2l);
+    float s = __builtin_amdgcn_exp2f(fmaf(b3, lt, -M));
+    if (MASK) s = (t > 0.0f) ? s : 0.0f;
+    const float dt = t - D;
+    float g1 = fmaf(cD * s, fmaf(-b2, dt, 1.0f), cF);     // dL/dt
+    float gd2 = fmaf(cDb * s, dt, cE * e);                // dL/dd2
+    if (ARGMAX) {
+      const float ex = (b3 * lt == M) ? pxc[p] : 0.0f;   // the pair holding the maximum logit
+      g1 = fmaf(-b2, ex, g1);
+      gd2 = fmaf(b3, ex, gd2);
+    }
+    const float Gq = gd2 * qq;
+    const float h = fmaf(Gq, X, g1 + g1);
+    const float gX = fmaf(-(t * t), h, fmaf(Gq, t, g1 * rr));
+    const float c = h * (t * rr);
+    const float c1 = c * Y1, c2 = c * Y2;
+    G[0] = fmaf(gX, xf, G[0]); G[1] = fmaf(gX, yf, G[1]); G[2] += gX;
+    G[3] = fmaf(c1, xf, G[3]); G[4] = fmaf(c1, yf, G[4]); G[5] += c1;
+    G[6] = fmaf(c2, xf, G[6]); G[7] = fmaf(c2, yf, G[7]); G[8] += c2;
+    G[9] += gd2;
+    G[10] = fmaf(gd2, Wr, G[10]);
+  }
+}
+
+// One block (4 waves) per tile; lane = one Gaussian of the tile's lists (phase 1 then
+// phase 2, or all Gaussians in dense mode).  The tile's per-pixel terms (x', y', M, D and
+// the cotangent factors) are staged once in LDS and read as wave-uniform broadcasts.
+//  * Views with blend records: the Gaussians go in chunks of 64, one lane each, and every
+//    wave takes the same chunk over its own quarter of the tile's pixels, so the 4 waves
+//    carry equal work whatever the list length.  Each lane keeps 11 sums in registers; the 4
+//    quarters meet in LDS and each (tile, Gaussian) adds its 11 sums to the per-(Gaussian,
+//    view) sums in the workspace with one atomic each.  The chain rule is linear in those
+//    sums, so k_bwd_chain applies it once per (Gaussian, view) afterwards.
+//  * Other views (degenerate records, b3 <= 0): the waves take chunks of 64 Gaussians over
+//    all pixels, with 13 sums per lane and the chain rule per (tile, Gaussian) straight into
+//    the scene gradients.
 template <int TS>
 __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   constexpr int NPIX = TS * TS;
   __shared__ float4 pxa[NPIX];   // x', y', M, D
   __shared__ float4 pxb[NPIX];   // cE, cD, b3 cD, 1e-20 cD
   __shared__ float pxc[NPIX];    // cX (maximum-logit term)
+  __shared__ float red[4][11][64];
 
   const int64_t gt = blockIdx.x;
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
@@ -276,43 +316,63 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   for (int i = threadIdx.x; i < npx; i += 256) {
     const int px = tx * TS + (i % wv), py = ty * TS + (i / wv);
     const int64_t lp = (int64_t)py * jb.W + px;
-    const int64_t pix = jb.pix_base + lp;
-    const float M = A.st[pix];
-    const float S = A.st[A.total_pix + pix];
-    const float Z = A.st[2 * A.total_pix + pix];
-    const float E = A.st[3 * A.total_pix + pix];
-    const float D = A.st[5 * A.total_pix + pix];
-    const float gD = bj.gD ? bj.gD[lp] : 0.0f;
-    const float gA = bj.gA ? bj.gA[lp] : 0.0f;
-    const float den = S + 1e-20f * ncnt;
-    float cE = gA * expf(-E);
-    float cD = (den > 0.0f) ? gD / den : 0.0f;
-    if (!(cE == cE)) cE = 0.0f;
-    if (!(cD == cD) || !(D == D)) cD = 0.0f;
-    const float xr = jb.ray_xy ? jb.ray_xy[2 * lp] : (float)px - jb.cx;
-    const float yr = jb.ray_xy ? jb.ray_xy[2 * lp + 1] : (float)py - jb.cy;
-    float cX = -cD * fmaf(-D, S, Z);
-    if (!(cX == cX) || fabsf(cX) > 3.0e38f) cX = 0.0f;
-    floor_px = floor_px || !(S >= 1e-11f * ncnt);
-    pxa[i] = make_float4(xr, yr, M, D);
-    pxb[i] = make_float4(cE, cD, jb.beta3 * cD, 1e-20f * cD);
-    pxc[i] = cX;
+    const PixTerms P = pix_terms(A, jb, bj, lp, px, py, ncnt);
+    floor_px = floor_px || P.floor_px;
+    pxa[i] = make_float4(P.xf, P.yf, P.M, P.D);
+    pxb[i] = make_float4(P.cE, P.cD, P.cDb, P.cF);
+    pxc[i] = P.cX;
   }
   const bool floor_tile = __syncthreads_or(floor_px ? 1 : 0) != 0;
 
   const float b3 = jb.beta3, b2 = jb.beta2;
   const float nb2l = -jb.beta2 * kLog2e;
-  const float kd = -0.5f * kLog2e;
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int32_t jflags = A.job_flags[ji];
+
+  if (use_rec2(jb, jflags)) {
+    const bool masked = A.dense || !A.tile_np || (A.tile_np[2 * gt] | A.tile_np[2 * gt + 1]) != 0;
+    const float kap = nb2l / b3;
+    const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
+    float* gsum = A.gsum + jb.rec_base;
+    const int q = (npx + 3) >> 2;
+    const int p0 = min(wave * q, npx), p1 = min(p0 + q, npx);
+    for (int64_t c0 = 0; c0 < total; c0 += 64) {
+      const int64_t kk = c0 + lane;
+      const bool act = kk < total;
+      int64_t gid = 0;
+      float G[11];
+#pragma unroll
+      for (int k = 0; k < 11; ++k) G[k] = 0.0f;
+      if (act) {
+        gid = A.dense ? kk : (int64_t)A.list[kk < n1 ? s1 + kk : s2 + (kk - n1)];
+        const float4 q0 = rec24[gid * 4 + 0];
+        const float4 q1 = rec24[gid * 4 + 1];
+        const float4 q2 = rec24[gid * 4 + 2];
+        const float k9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
+        if (masked && floor_tile) bwd_pixels_rec2<true, true>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
+        else if (masked) bwd_pixels_rec2<true, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
+        else bwd_pixels_rec2<false, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
+      }
+#pragma unroll
+      for (int k = 0; k < 11; ++k) red[wave][k][lane] = G[k];
+      __syncthreads();
+      // wave w adds components k = w, w+4, w+8 of the chunk's 64 Gaussians
+      if (act) {
+        for (int k = wave; k < 11; k += 4) {
+          const float v = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
+          atomicAdd(gsum + (int64_t)k * A.total_rec + gid, v);
+        }
+      }
+      __syncthreads();
+    }
+    return;
+  }
+
+  // ---- direct form (views without blend records) ----
+  const float kd = -0.5f * kLog2e;
   const bool safe = (jflags & kFlagDegenerate) != 0;   // the view has degenerate records
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
   const float ifx = 1.0f / jb.fx, ify = 1.0f / jb.fy;
-  const bool r2 = !(jflags & (kFlagDegenerate | kFlagNoBlendRec)) && jb.beta3 > 0.0f && jb.beta3 < 3.0e38f &&
-                  (jb.beta2 == jb.beta2) && fabsf(jb.beta2) < 3.0e38f;
-  const bool masked = A.dense || !A.tile_np || (A.tile_np[2 * gt] | A.tile_np[2 * gt + 1]) != 0;
-  const float kap = nb2l / b3;
-  const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
 
   for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 256) {
     const int64_t kk = c0 + lane;
@@ -320,21 +380,6 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     int64_t gid;
     if (A.dense) gid = kk;
     else gid = (kk < n1) ? (int64_t)A.list[s1 + kk] : (int64_t)A.list[s2 + (kk - n1)];
-    if (r2) {
-      const float4 q0 = rec24[gid * 4 + 0];
-      const float4 q1 = rec24[gid * 4 + 1];
-      const float4 q2 = rec24[gid * 4 + 2];
-      const float kk9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
-      float GX[3] = {0.f, 0.f, 0.f}, GY1[3] = {0.f, 0.f, 0.f}, GY2[3] = {0.f, 0.f, 0.f}, Gw = 0.f, Gqq = 0.f;
-      if (masked && floor_tile)
-        bwd_pixels_rec2<true, true>(pxa, pxb, pxc, npx, kk9, q2.y, q2.z, q2.w, b3, b2, kap, GX, GY1, GY2, Gw, Gqq);
-      else if (masked)
-        bwd_pixels_rec2<true, false>(pxa, pxb, pxc, npx, kk9, q2.y, q2.z, q2.w, b3, b2, kap, GX, GY1, GY2, Gw, Gqq);
-      else
-        bwd_pixels_rec2<false, false>(pxa, pxb, pxc, npx, kk9, q2.y, q2.z, q2.w, b3, b2, kap, GX, GY1, GY2, Gw, Gqq);
-      rec2_chain(jb, bj, gid, GX, GY1, GY2, Gw, Gqq);
-      continue;
-    }
     const float4 r0 = rec4[gid * 4 + 0];
     const float4 r1 = rec4[gid * 4 + 1];
     const float4 r2 = rec4[gid * 4 + 2];
@@ -351,9 +396,9 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     // chain rule for this (tile, Gaussian) pair, straight into the scene gradients:
     //   dM = [G_a/fx, G_b/fy, G_g],  dA = dM R + G_q p^T (lower part),  dmu = A^T G_q,
     //   d(prec)[k][i] += dA[i][k] for k <= i,  dw += G_w
-    const float* P = jb.prec + 9 * gid;
+    const float* Pp = jb.prec + 9 * gid;
     const float* mu = jb.means + 3 * gid;
-    const float u00 = P[0], u01 = P[1], u02 = P[2], u11 = P[4], u12 = P[5], u22 = P[8];
+    const float u00 = Pp[0], u01 = Pp[1], u02 = Pp[2], u11 = Pp[4], u12 = Pp[5], u22 = Pp[8];
     const float pv[3] = {mu[0] - jb.o[0], mu[1] - jb.o[1], mu[2] - jb.o[2]};
     const float dm[3][3] = {{Ga0 * ifx, Gb0 * ify, Gg0}, {Ga1 * ifx, Gb1 * ify, Gg1}, {Ga2 * ifx, Gb2 * ify, Gg2}};
     const float gq[3] = {Gq0, Gq1, Gq2};
@@ -382,6 +427,33 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   }
 }
 
+// Zero the per-(Gaussian, view) sums before the backward (a kernel, not hipMemset: graph-safe).
+__global__ void k_fill_zero(float4* p, int64_t n4) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n4; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Chain rule once per (Gaussian, view) record of a view with blend records.
+__global__ __launch_bounds__(256) void k_bwd_chain(const Job* __restrict__ jobs, const BwdJob* __restrict__ bjobs,
+                                                   int n_jobs, const int32_t* __restrict__ job_flags,
+                                                   const float* __restrict__ gsum, int64_t total_rec) {
+  const int64_t r = blockIdx.x * 256ll + threadIdx.x;
+  if (r >= total_rec) return;
+  const int ji = find_job(n_jobs, r, [&](int k) { return jobs[k].rec_base; });
+  const Job& jb = jobs[ji];
+  const int64_t j = r - jb.rec_base;
+  if (j >= jb.n || !use_rec2(jb, job_flags[ji])) return;
+  float g[kGsumPlanes];
+  bool any = false;
+#pragma unroll
+  for (int k = 0; k < kGsumPlanes; ++k) {
+    g[k] = gsum[(int64_t)k * total_rec + r];
+    any = any || g[k] != 0.0f;
+  }
+  if (!any) return;   // no tile listed this Gaussian
+  rec2_chain(jb, bjobs[ji], j, g, g + 3, g + 6, g[9], g[10]);
+}
+
 }  // namespace fe
 
 // ------------------------------------------------------------------------------------------
@@ -398,17 +470,28 @@ static hipError_t bwd_t(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
                             int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                             const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
-                            const float* st, int64_t total_pix, const int32_t* job_flags, int dense) {
-  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, rec2, tile_np, off1, off2, list, nullptr,
-            nullptr, st, total_pix, job_flags, dense};
+                            const float* st, int64_t total_pix, const int32_t* job_flags, float* gsum,
+                            int64_t total_rec, int dense) {
+  const int64_t n4 = (total_rec * kGsumPlanes + 3) / 4;   // the region is padded to 256 B
+  if (n4 > 0)
+    hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((n4 + 255) / 256, 4096)), dim3(256), 0, s,
+                       reinterpret_cast<float4*>(gsum), n4);
+  BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, rec2, tile_np, off1, off2, list, st,
+            total_pix, job_flags, gsum, total_rec, dense};
+  hipError_t e = hipErrorInvalidValue;
   switch (ts) {
-    case 8: return bwd_t<8>(s, total_tiles, a);
-    case 16: return bwd_t<16>(s, total_tiles, a);
-    case 32: return bwd_t<32>(s, total_tiles, a);
+    case 8: e = bwd_t<8>(s, total_tiles, a); break;
+    case 16: e = bwd_t<16>(s, total_tiles, a); break;
+    case 32: e = bwd_t<32>(s, total_tiles, a); break;
   }
-  return hipErrorInvalidValue;
+  if (e != hipSuccess) return e;
+  if (total_rec > 0)
+    hipLaunchKernelGGL(k_bwd_chain, dim3((unsigned)((total_rec + 255) / 256)), dim3(256), 0, s, jobs,
+                       reinterpret_cast<const BwdJob*>(bjobs), n_jobs, job_flags, gsum, total_rec);
+  return hipGetLastError();
 }
 
 size_t bwd_job_bytes() { return sizeof(BwdJob); }
+size_t bwd_gsum_floats() { return kGsumPlanes; }
 
 }  // namespace fe

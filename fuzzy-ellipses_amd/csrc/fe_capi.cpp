@@ -49,7 +49,7 @@ struct Layout {
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_rec2 = 0, o_tnp = 0,
          o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
-         o_bjobs = 0, o_rect = 0, bytes = 0;
+         o_bjobs = 0, o_gsum = 0, o_rect = 0, bytes = 0;
 };
 
 // A small header at the start of the workspace lets fe_read_stats find the counters.
@@ -210,6 +210,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   }
   if (L->save) {
     L->o_bjobs = p; p = align256(p + sizeof(BwdJob) * n_views);
+    L->o_gsum = p; p = align256(p + sizeof(float) * bwd_gsum_floats() * (size_t)std::max<int64_t>(rec, 1));
   }
   if (L->has_rays) {
     L->o_rect = p; p = align256(p + sizeof(float) * 4 * tiles);
@@ -434,7 +435,7 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off1),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off2),
                             dense ? nullptr : at<uint32_t>(ws, L.o_list), at<float>(ws, L.o_state), L.total_pix,
-                            at<int32_t>(ws, L.o_flags), dense ? 1 : 0);
+                            at<int32_t>(ws, L.o_flags), at<float>(ws, L.o_gsum), L.total_rec, dense ? 1 : 0);
   }));
   return FE_OK;
 }

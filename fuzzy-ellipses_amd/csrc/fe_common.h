@@ -115,6 +115,13 @@ struct Counters {
   unsigned long long pad[4];
 };
 
+// Whether a view's blends use the blend records: no degenerate record, every record valid,
+// and b3 > 0 (the running maximum is taken over l/b3).
+__device__ __forceinline__ bool use_rec2(const Job& jb, int32_t flags) {
+  return !(flags & (kFlagDegenerate | kFlagNoBlendRec)) && jb.beta3 > 0.0f && jb.beta3 < 3.0e38f &&
+         (jb.beta2 == jb.beta2) && fabsf(jb.beta2) < 3.0e38f;
+}
+
 __device__ __forceinline__ float dot3(const float* x, const float* y) {
   return fmaf(x[0], y[0], fmaf(x[1], y[1], x[2] * y[2]));
 }

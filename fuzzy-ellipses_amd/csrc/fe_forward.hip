@@ -646,13 +646,6 @@ __device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, flo
   }
 }
 
-// Whether a view's blends use the blend records: no degenerate record, every record valid,
-// and b3 > 0 (the running maximum is taken over l/b3).
-__device__ __forceinline__ bool use_rec2(const Job& jb, int32_t flags) {
-  return !(flags & (kFlagDegenerate | kFlagNoBlendRec)) && jb.beta3 > 0.0f && jb.beta3 < 3.0e38f &&
-         (jb.beta2 == jb.beta2) && fabsf(jb.beta2) < 3.0e38f;
-}
-
 struct BlendArgs {
   const Job* jobs;
   int n_jobs;

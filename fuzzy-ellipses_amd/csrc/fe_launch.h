@@ -40,7 +40,9 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
                             int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                             const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
-                            const float* st, int64_t total_pix, const int32_t* job_flags, int dense);
+                            const float* st, int64_t total_pix, const int32_t* job_flags, float* gsum,
+                            int64_t total_rec, int dense);
+size_t bwd_gsum_floats();
 
 hipError_t launch_sil_loss(hipStream_t s, const fe_sil_view* views, int n_views, float* cnt_part,
                            float* loss_part, float w_depth, float clip, float* loss);
