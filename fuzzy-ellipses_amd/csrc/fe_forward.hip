@@ -167,11 +167,13 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
 
 // ------------------------------------------------------------------------------------------
 // (b) tile membership.  One block (4 waves) per super-tile of sf x sf tiles; each wave tests
-// 64 Gaussians at a time.  A lane loads its Gaussian's record once, bounds it over the whole
-// super-tile and, only if that bound keeps it, over each tile of the group (a pair the
-// group bound rejects is rejected for every tile in it: the group bound is a bound over a
-// superset of rays).  Each tile's decisions are ballot-packed into 64-bit membership words
-// and counted into the tile histogram.
+// 64 Gaussians at a time.  A lane loads its Gaussian's record once and bounds it over the
+// whole super-tile (a pair the group bound rejects is rejected for every tile in it: the
+// group bound is a bound over a superset of rays).  The survivors go to an LDS queue, and
+// the per-tile bounds run on the queue with every lane busy (about 1 Gaussian in 10
+// survives, so testing them in their original waves would leave most lanes idle).  Each
+// tile's decisions are set as bits of 64-bit membership words held in LDS for a window of
+// kWinWords words (4096 Gaussians), then written out and counted into the tile histogram.
 // PHASE 1 keeps pairs whose density bound reaches exp(-c_alpha):  d0_lo/2 - w <= c_alpha,
 //   writing mask1.
 // PHASE 2 keeps pairs absent from mask1 whose depth-logit bound reaches the tile threshold
@@ -180,9 +182,9 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
 //   such a pair could raise the max logit so far that the 1e-20 weight floor becomes visible
 //   (ubm - lmin >= floor_thr), the tile is completed densely (every remaining pair).
 // ------------------------------------------------------------------------------------------
-constexpr int kMaxSF = 4;
+constexpr int kWinWords = 64;   // membership words per LDS window
 
-template <int PHASE>
+template <int PHASE, int SFMAX>
 __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs, int n_jobs, int ts,
                                                    const float* __restrict__ rec,
                                                    unsigned long long* __restrict__ mask1,
@@ -193,9 +195,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
                                                    uint32_t* __restrict__ tile_np,
                                                    const int32_t* __restrict__ job_flags,
                                                    Counters* __restrict__ cnt_out) {
-  constexpr int NS = kMaxSF * kMaxSF;
-  __shared__ uint32_t sh_cnt[4][NS];
-  __shared__ uint32_t sh_np[4][NS];
+  constexpr int NS = SFMAX * SFMAX;   // sub-tiles per group at most (ray-list views use 1)
   __shared__ float sh_ubm[4][NS];
   const int64_t gs = blockIdx.x;
   const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
@@ -267,81 +267,165 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
   }
   __syncthreads();
   const float tau_s = (PHASE == 2) ? tau_grp : 0.0f;
-  uint32_t cnt[NS];
-  uint32_t npbits = 0;   // bit k: sub-tile k lists a pair not known to be in front of the camera
+  __shared__ unsigned long long wmask[NS][kWinWords];   // this window's membership words
+  __shared__ unsigned long long wlist[NS][kWinWords];   // phase 2: the phase-1 words of the window
+  __shared__ uint32_t queue[kWinWords * 64];            // group survivors (Gaussian ids)
+  __shared__ uint32_t qn;
+  __shared__ uint32_t tcnt[NS], tnp[NS];
+  if (threadIdx.x < NS) { tcnt[threadIdx.x] = 0; tnp[threadIdx.x] = 0; }
   float ubm[NS];
 #pragma unroll
-  for (int k = 0; k < NS; ++k) { cnt[k] = 0; ubm[k] = -INFINITY; }
+  for (int k = 0; k < NS; ++k) ubm[k] = -INFINITY;
+  const bool single = (nsub == 1);   // the group is the tile: its bound decides directly
 
-  for (int c = wave; c < jb.nw; c += 4) {
-    const int j = c * 64 + lane;
-    float r[16];
-    bool valid = j < jb.n;
-    if (valid) {
-      const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float4 t4 = r4[i];
-        r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
-      }
-    }
-    const bool degenerate = valid && ((__float_as_int(r[15]) & 1) != 0);
-    const bool forced = valid && (dense_job || degenerate);   // kept in phase 1, never in phase 2
-    // group test
-    bool keep_s = false, plus_s = true;
-    float ubm_s = -INFINITY;
-    if (valid && !forced) {
-      const PairBound pb = pair_bound(r, sg);
-      const float w = r[12];
-      if (PHASE == 1) {
-        keep_s = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
-      } else {
-        const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
-        const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
-        keep_s = pb.unbounded || (ub >= tau_s);
-        plus_s = pb.plus;
-        if (!pb.plus) ubm_s = fmaf(b3l, dlo, -(b2l * pb.t_any_lb));
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      if (k < nsub) {
+  for (int w0 = 0; w0 < jb.nw; w0 += kWinWords) {
+    const int nww = min(kWinWords, jb.nw - w0);
+    for (int i = threadIdx.x; i < NS * kWinWords; i += 256) {
+      const int k = i / kWinWords, c = i - k * kWinWords;
+      wmask[k][c] = 0ull;
+      unsigned long long lw = 0ull;
+      if (PHASE == 2 && k < nsub && c < nww) {
         const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
-        const int64_t row = jb.mask_base + tile * jb.nw;
-        const unsigned long long old = (PHASE == 2) ? mask1[row + c] : 0ull;
-        const bool listed = (PHASE == 2) && ((old >> lane) & 1ull);
-        bool keep = false, front = false;
-        if (PHASE == 1) {
-          keep = forced;
+        lw = mask1[jb.mask_base + tile * jb.nw + w0 + c];
+      }
+      wlist[k][c] = lw;
+    }
+    if (threadIdx.x == 0) qn = 0;
+    __syncthreads();
+
+    // pass 1: group bound of every Gaussian of the window (lane = Gaussian)
+    for (int c = wave; c < nww; c += 4) {
+      const int j = (w0 + c) * 64 + lane;
+      const bool valid = j < jb.n;
+      float r[16];
+      if (valid) {
+        const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float4 t4 = r4[i];
+          r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
         }
-        if (valid && !forced && !listed) {
-          if (keep_s) {
-            const PairBound pb = pair_bound(r, tg[k]);
-            const float w = r[12];
-            front = pb.front;
-            if (PHASE == 1) {
-              keep = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
-            } else {
-              const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
-              const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
-              keep = pb.unbounded || (ub >= tau[k]);
-              if (!keep && !pb.plus) ubm[k] = fmaxf(ubm[k], fmaf(b3l, dlo, -(b2l * pb.t_any_lb)));
-            }
-          } else if (PHASE == 2 && !plus_s) {
-            ubm[k] = fmaxf(ubm[k], ubm_s);
+      }
+      const bool degenerate = valid && ((__float_as_int(r[15]) & 1) != 0);
+      const bool forced = valid && (dense_job || degenerate);   // kept in phase 1, never in phase 2
+      bool keep_s = false, plus_s = true, front_s = false;
+      float ubm_s = -INFINITY;
+      bool keep1 = false;   // single-tile groups: the decision itself
+      if (valid && !forced) {
+        const PairBound pb = pair_bound(r, sg);
+        const float w = r[12];
+        front_s = pb.front;
+        if (PHASE == 1) {
+          keep_s = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
+        } else {
+          const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
+          const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
+          keep_s = pb.unbounded || (ub >= tau_s);
+          plus_s = pb.plus;
+          if (!pb.plus) ubm_s = fmaf(b3l, dlo, -(b2l * pb.t_any_lb));
+        }
+        keep1 = keep_s;
+      }
+      if (PHASE == 2 && valid && !forced) {
+        // pairs listed in phase 1 are skipped; a dropped pair that may lie behind the camera
+        // raises the tile's ubm (here from the group bound, which covers every sub-tile)
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          if (k < nsub) {
+            const bool listed = (wlist[k][c] >> lane) & 1ull;
+            if (!listed && !keep_s && !plus_s) ubm[k] = fmaxf(ubm[k], ubm_s);
           }
         }
+      }
+      if (single) {
+        // the group bound is the tile bound
+        const bool keep = (PHASE == 1) ? (forced || keep1) : (keep1 && !((wlist[0][c] >> lane) & 1ull));
         const unsigned long long word = __ballot(keep);
+        const unsigned long long npw = __ballot(keep && !front_s);   // forced pairs: front_s false
         if (lane == 0) {
-          if (PHASE == 1) mask1[row + c] = word;
-          else mask2[row + c] = word;
+          wmask[0][c] = word;
+          if (npw != 0ull) tnp[0] = 1;
         }
-        cnt[k] += (uint32_t)__popcll(word);
-        // listed pairs not provably in front of the camera on the whole tile
-        if (__ballot(keep && !front) != 0ull) npbits |= 1u << k;
+        continue;
+      }
+      if (PHASE == 1) {
+        const unsigned long long fw = __ballot(forced);
+        if (fw != 0ull && lane == 0) {
+          for (int k = 0; k < nsub; ++k) { wmask[k][c] = fw; tnp[k] = 1; }
+        }
+      }
+      bool need = keep_s;
+      if (PHASE == 2 && keep_s) {
+        // a Gaussian listed in phase 1 on every sub-tile has nothing left to test
+        bool all_listed = true;
+#pragma unroll
+        for (int k = 0; k < NS; ++k)
+          if (k < nsub) all_listed = all_listed && ((wlist[k][c] >> lane) & 1ull);
+        need = !all_listed;
+      }
+      const unsigned long long sw = __ballot(need);
+      if (sw != 0ull) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&qn, (uint32_t)__popcll(sw));
+        base = __shfl(base, 0, 64);
+        const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        if (need) queue[base + __popcll(sw & lt)] = (uint32_t)j;
       }
     }
+    __syncthreads();
+
+    // pass 2: per-tile bounds of the survivors, one survivor per lane
+    if (!single) {
+      const uint32_t nq = qn;
+      for (uint32_t i = threadIdx.x; i < nq; i += 256) {
+        const int j = (int)queue[i];
+        const int c = (j >> 6) - w0;
+        const int bit = j & 63;
+        float r[16];
+        const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const float4 t4 = r4[q];
+          r[4 * q] = t4.x; r[4 * q + 1] = t4.y; r[4 * q + 2] = t4.z; r[4 * q + 3] = t4.w;
+        }
+        const float w = r[12];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          if (k >= nsub) continue;
+          if (PHASE == 2 && ((wlist[k][c] >> bit) & 1ull)) continue;   // listed in phase 1
+          const PairBound pb = pair_bound(r, tg[k]);
+          bool keep;
+          if (PHASE == 1) {
+            keep = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
+          } else {
+            const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
+            const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
+            keep = pb.unbounded || (ub >= tau[k]);
+            if (!keep && !pb.plus) ubm[k] = fmaxf(ubm[k], fmaf(b3l, dlo, -(b2l * pb.t_any_lb)));
+          }
+          if (keep) {
+            atomicOr(&wmask[k][c], 1ull << bit);
+            if (!pb.front) tnp[k] = 1;
+          }
+        }
+      }
+    }
+    __syncthreads();
+
+    // write the window's words out and count them
+    for (int i = threadIdx.x; i < nsub * kWinWords; i += 256) {
+      const int k = i / kWinWords, c = i - k * kWinWords;
+      if (c < nww) {
+        const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
+        const unsigned long long word = wmask[k][c];
+        if (PHASE == 1) mask1[jb.mask_base + tile * jb.nw + w0 + c] = word;
+        else mask2[jb.mask_base + tile * jb.nw + w0 + c] = word;
+        if (word != 0ull) atomicAdd(&tcnt[k], (uint32_t)__popcll(word));
+      }
+    }
+    __syncthreads();
   }
+
 #pragma unroll
   for (int k = 0; k < NS; ++k) {
     if (PHASE == 2) {
@@ -349,16 +433,16 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
       for (int o = 32; o > 0; o >>= 1) u = fmaxf(u, __shfl_xor(u, o, 64));
       ubm[k] = u;
     }
-    if (lane == 0) { sh_cnt[wave][k] = cnt[k]; sh_np[wave][k] = (npbits >> k) & 1u; sh_ubm[wave][k] = ubm[k]; }
+    if (lane == 0) sh_ubm[wave][k] = ubm[k];
   }
   __syncthreads();
-  // one wave finishes the sub-tiles: lane k owns sub-tile k
+  // one wave finishes the sub-tiles
   if (wave == 0) {
     for (int k = 0; k < nsub; ++k) {
       const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
       const int64_t gt = jb.tile_base + tile;
-      uint32_t total = sh_cnt[0][k] + sh_cnt[1][k] + sh_cnt[2][k] + sh_cnt[3][k];
-      uint32_t np = sh_np[0][k] | sh_np[1][k] | sh_np[2][k] | sh_np[3][k];
+      uint32_t total = tcnt[k];
+      uint32_t np = tnp[k];
       if (PHASE == 2) {
         const float ubm_t = fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k]));
         const bool dense_tile = (ubm_t - lmin[k] >= jb.floor_thr);   // false when nothing in doubt
@@ -871,12 +955,23 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
                             int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
                             uint32_t* count, const float* tile_lmin, float* tile_ubm, uint32_t* tile_np,
                             const int32_t* job_flags, Counters* cnt) {
-  if (phase == 1)
-    hipLaunchKernelGGL(k_tile_mask<1>, dim3((unsigned)total_stiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
-                       mask2, count, tile_lmin, tile_ubm, tile_np, job_flags, cnt);
-  else
-    hipLaunchKernelGGL(k_tile_mask<2>, dim3((unsigned)total_stiles), dim3(256), 0, s, jobs, n_jobs, ts, rec, mask1,
-                       mask2, count, tile_lmin, tile_ubm, tile_np, job_flags, cnt);
+  // the group factor is 4 for 8-pixel tiles, 2 for 16, 1 for 32 and for ray lists
+  const dim3 g((unsigned)total_stiles), b(256);
+  if (phase == 1) {
+    if (ts == 8) hipLaunchKernelGGL((k_tile_mask<1, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
+                                    tile_ubm, tile_np, job_flags, cnt);
+    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<1, 2>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
+                                          tile_lmin, tile_ubm, tile_np, job_flags, cnt);
+    else hipLaunchKernelGGL((k_tile_mask<1, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
+                            tile_ubm, tile_np, job_flags, cnt);
+  } else {
+    if (ts == 8) hipLaunchKernelGGL((k_tile_mask<2, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
+                                    tile_ubm, tile_np, job_flags, cnt);
+    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 2>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
+                                          tile_lmin, tile_ubm, tile_np, job_flags, cnt);
+    else hipLaunchKernelGGL((k_tile_mask<2, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
+                            tile_ubm, tile_np, job_flags, cnt);
+  }
   return hipGetLastError();
 }
 
