@@ -336,18 +336,31 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     float* gsum = A.gsum + jb.rec_base;
     const int q = (npx + 3) >> 2;
     const int p0 = min(wave * q, npx), p1 = min(p0 + q, npx);
-    for (int64_t c0 = 0; c0 < total; c0 += 64) {
+    // The next chunk's records are loaded before this chunk's atomics are issued: stores
+    // and atomics count in the same wait counter as loads, and waiting for records loaded
+    // after the atomics would wait for the atomics to complete at memory.
+    int64_t gid_n = 0;
+    float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa, qc = qa;
+    auto fetch = [&](int64_t c0) {
       const int64_t kk = c0 + lane;
-      const bool act = kk < total;
-      int64_t gid = 0;
+      if (kk < total) {
+        gid_n = A.dense ? kk : (int64_t)A.list[kk < n1 ? s1 + kk : s2 + (kk - n1)];
+        qa = rec24[gid_n * 4 + 0];
+        qb = rec24[gid_n * 4 + 1];
+        qc = rec24[gid_n * 4 + 2];
+      }
+    };
+    fetch(0);
+    asm volatile("" ::"v"(qa.x), "v"(qa.y), "v"(qa.z), "v"(qa.w), "v"(qb.x), "v"(qb.y), "v"(qb.z), "v"(qb.w),
+                 "v"(qc.x), "v"(qc.y), "v"(qc.z), "v"(qc.w));
+    for (int64_t c0 = 0; c0 < total; c0 += 64) {
+      const bool act = c0 + lane < total;
+      const int64_t gid = gid_n;
+      const float4 q0 = qa, q1 = qb, q2 = qc;
       float G[11];
 #pragma unroll
       for (int k = 0; k < 11; ++k) G[k] = 0.0f;
       if (act) {
-        gid = A.dense ? kk : (int64_t)A.list[kk < n1 ? s1 + kk : s2 + (kk - n1)];
-        const float4 q0 = rec24[gid * 4 + 0];
-        const float4 q1 = rec24[gid * 4 + 1];
-        const float4 q2 = rec24[gid * 4 + 2];
         const float k9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
         if (masked && floor_tile) bwd_pixels_rec2<true, true>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
         else if (masked) bwd_pixels_rec2<true, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
@@ -356,12 +369,21 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
 #pragma unroll
       for (int k = 0; k < 11; ++k) red[wave][k][lane] = G[k];
       __syncthreads();
+      if (c0 + 64 < total) fetch(c0 + 64);
       // wave w adds components k = w, w+4, w+8 of the chunk's 64 Gaussians
-      if (act) {
-        for (int k = wave; k < 11; k += 4) {
-          const float v = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
-          atomicAdd(gsum + (int64_t)k * A.total_rec + gid, v);
-        }
+      float v[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int k = min(wave + 4 * i, 10);
+        v[i] = red[0][k][lane] + red[1][k][lane] + red[2][k][lane] + red[3][k][lane];
+      }
+      // the next records must be in registers before the atomics go out (see above)
+      asm volatile("" ::"v"(qa.x), "v"(qa.y), "v"(qa.z), "v"(qa.w), "v"(qb.x), "v"(qb.y), "v"(qb.z), "v"(qb.w),
+                   "v"(qc.x), "v"(qc.y), "v"(qc.z), "v"(qc.w));
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const int k = wave + 4 * i;
+        if (act && k < 11) atomicAdd(gsum + (int64_t)k * A.total_rec + gid, v[i]);
       }
       __syncthreads();
     }
