@@ -22,8 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.join(ROOT, "fuzzy-ellipses_amd")
 
 METRIC = "ray-Gaussian evals/s + fwd+bwd frames/s @480x640,N=4096; served p50/p99 ms"
-FP32_PEAK_TFLOPS = 157.3          # MI355X published dense FP32 (packed)
-FP32_PEAK_UNPACKED_TFLOPS = 78.6  # without packed FP32 (this build disables packed FP32)
+FP32_PEAK_TFLOPS = 157.3          # MI355X published dense FP32 vector peak
 FLOPS_FWD = 36                    # per evaluated (pixel, Gaussian) pair, blend forward
 FLOPS_BWD = 93                    # per evaluated pair, blend backward incl. recompute
 
@@ -241,7 +240,7 @@ def workload_gmm(fea, sc, dev, steps):
             "fwd_ms": round(fwd_s * 1e3, 4), "bwd_ms": round(bwd_s * 1e3, 4),
             "fwd_tflops": round(GMM_FLOPS_FWD * pairs / fwd_s / 1e12, 3),
             "bwd_tflops": round(GMM_FLOPS_BWD * pairs / bwd_s / 1e12, 3),
-            "peak_fp32_unpacked_tflops": FP32_PEAK_UNPACKED_TFLOPS,
+            "peak_fp32_tflops": FP32_PEAK_TFLOPS,
             "config": {"workload": "GMM log-likelihood of 2^20 uniform points under 4096 Gaussians"}}
 
 
@@ -371,8 +370,7 @@ def main():
     achieved = flops_per_launch[dom] / avg_s / 1e12 if avg_s > 0 else 0.0
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, args), "kernel": dom,
-            "note": "compute-bound on FP32 VALU (no MFMA); frac of unpacked FP32 peak 78.6 = "
-                    f"{achieved / FP32_PEAK_UNPACKED_TFLOPS:.4f}; flops/launch = "
+            "note": "compute-bound on FP32 VALU (no MFMA, no packed FP32); flops/launch = "
                     f"{FLOPS_FWD if dom != 'bwd_blend' else FLOPS_BWD} x evaluated pairs"}
 
     out = {

@@ -679,53 +679,66 @@ __device__ __forceinline__ void blend_batch(const float4* lds, int nbr, float xf
   }
 }
 
-// The same batch through blend records (fe_common.h): per pair 3 linear forms, one
+// The same batch through blend records (fe_common.h), for the PPT pixels of this thread:
+// each record is read from LDS once for all of them.  Per pair 3 linear forms, one
 // reciprocal and two exponentials.  Logits are kept as lt = l/b3 (log2 units, b3 > 0), so
-// the running maximum Mt is of lt and the reference M = b3*Mt is what S and Z are relative to
-// (the value saved for the second pass and the backward).  MASK applies the t* > 0 test; a
-// tile whose listed pairs are all in front of the camera skips it.
-template <int SUB, bool MASK>
-__device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, float xf, float yf, float b3, float kap,
-                                                 float& M, float& Mt, float& S, float& Z, float& E, float& Zall) {
+// the running maximum Mt is of lt and the reference M = b3*Mt is what S and Z are relative
+// to (the value saved for the second pass and the backward).  MASK applies the t* > 0
+// test; a tile whose listed pairs are all in front of the camera skips it.
+template <int SUB, bool MASK, int PPT>
+__device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, const float* xf, const float* yf, float b3,
+                                                 float kap, float* M, float* Mt, float* S, float* Z, float* E,
+                                                 float* Zall) {
   for (int k0 = 0; k0 < nbr; k0 += SUB) {
-    float ls[SUB], tsv[SUB];
-    float mx = Mt;
+    float ls[PPT][SUB], tsv[PPT][SUB];
+    float mx[PPT];
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) mx[q] = Mt[q];
 #pragma unroll
     for (int s = 0; s < SUB; ++s) {
       const float4 r0 = lds[(k0 + s) * 3 + 0];
       const float4 r1 = lds[(k0 + s) * 3 + 1];
       const float4 r2 = lds[(k0 + s) * 3 + 2];
       // kX = r0.xyz, kY1 = (r0.w, r1.x, r1.y), kY2 = (r1.z, r1.w, r2.x), wL = r2.y, cG = r2.z
-      const float X = fmaf(r0.x, xf, fmaf(r0.y, yf, r0.z));
-      const float Y1 = fmaf(r0.w, xf, fmaf(r1.x, yf, r1.y));
-      const float Y2 = fmaf(r1.z, xf, fmaf(r1.w, yf, r2.x));
-      const float W = fmaf(Y1, Y1, Y2 * Y2);
-      const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
-      const float t = X * rr;
-      // log2(e) * d2, rounded exactly as the backward recomputes it (it finds the pair
-      // holding the maximum logit by equality)
-      const float d2l = fmaf(-r2.z, W * rr, r2.y);
-      E += __builtin_amdgcn_exp2f(d2l);
-      const float lt = fmaf(kap, t, d2l);               // log2(e) * (b3 d2 - b2 t) / b3
-      Zall += t;
-      ls[s] = lt;
-      tsv[s] = t;
-      mx = fmaxf(mx, lt);
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const float X = fmaf(r0.x, xf[q], fmaf(r0.y, yf[q], r0.z));
+        const float Y1 = fmaf(r0.w, xf[q], fmaf(r1.x, yf[q], r1.y));
+        const float Y2 = fmaf(r1.z, xf[q], fmaf(r1.w, yf[q], r2.x));
+        const float W = fmaf(Y1, Y1, Y2 * Y2);
+        const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
+        const float t = X * rr;
+        // log2(e) * d2, rounded exactly as the backward recomputes it (it finds the pair
+        // holding the maximum logit by equality)
+        const float d2l = fmaf(-r2.z, W * rr, r2.y);
+        E[q] += __builtin_amdgcn_exp2f(d2l);
+        const float lt = fmaf(kap, t, d2l);               // log2(e) * (b3 d2 - b2 t) / b3
+        Zall[q] += t;
+        ls[q][s] = lt;
+        tsv[q][s] = t;
+        mx[q] = fmaxf(mx[q], lt);
+      }
     }
-    if (mx > Mt) {
-      const float Mn = b3 * mx;
-      const float sc = __builtin_amdgcn_exp2f(M - Mn);
-      S *= sc;
-      Z *= sc;
-      M = Mn;
-      Mt = mx;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      if (mx[q] > Mt[q]) {
+        const float Mn = b3 * mx[q];
+        const float sc = __builtin_amdgcn_exp2f(M[q] - Mn);
+        S[q] *= sc;
+        Z[q] *= sc;
+        M[q] = Mn;
+        Mt[q] = mx[q];
+      }
     }
 #pragma unroll
     for (int s = 0; s < SUB; ++s) {
-      float sv = __builtin_amdgcn_exp2f(fmaf(b3, ls[s], -M));
-      if (MASK) sv = (tsv[s] > 0.0f) ? sv : 0.0f;
-      S += sv;
-      Z = fmaf(sv, tsv[s], Z);
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        float sv = __builtin_amdgcn_exp2f(fmaf(b3, ls[q][s], -M[q]));
+        if (MASK) sv = (tsv[q][s] > 0.0f) ? sv : 0.0f;
+        S[q] += sv;
+        Z[q] = fmaf(sv, tsv[q][s], Z[q]);
+      }
     }
   }
 }
@@ -748,10 +761,20 @@ struct BlendArgs {
   int save;
 };
 
+// Pixels per thread of the blend: two for 16- and 32-pixel tiles (each record read from LDS
+// serves two pixels; with one, three 16-byte LDS broadcasts per pair bound the kernel), one
+// for 8-pixel tiles (whose 64 pixels fill one wave).
+template <int TS>
+struct BlendShape {
+  static constexpr int PPT = (TS >= 16) ? 2 : 1;
+  static constexpr int NT = TS * TS / PPT;   // threads per block
+};
+
 template <int TS, int GB, int PHASE>
-__global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
-  constexpr int NT = TS * TS;
-  constexpr int SUB = 8;
+__global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
+  constexpr int PPT = BlendShape<TS>::PPT;
+  constexpr int NT = BlendShape<TS>::NT;
+  constexpr int SUB = 8 / PPT;   // records per online-max update: 8 pairs in flight per thread
   __shared__ float4 lds[GB * 4];
   __shared__ float sh_red[NT / 64 + 1];
 
@@ -760,18 +783,25 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   const Job& jb = A.jobs[ji];
   const int tile = (int)(gt - jb.tile_base);
   const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
-  const int px = tx * TS + (threadIdx.x % TS);
-  const int py = ty * TS + (threadIdx.x / TS);
-  const int64_t lp = (int64_t)py * jb.W + px;
-  const bool inimg = (px < jb.W) && (py < jb.H) && (lp < jb.npix);
-  const int64_t pix = jb.pix_base + (inimg ? lp : 0);
-  float xf, yf;
-  if (jb.ray_xy) {
-    xf = inimg ? jb.ray_xy[2 * lp] : 0.0f;
-    yf = inimg ? jb.ray_xy[2 * lp + 1] : 0.0f;
-  } else {
-    xf = (float)px - jb.cx;
-    yf = (float)py - jb.cy;
+  // pixel q of this thread: tile pixel index threadIdx.x + q*NT
+  bool inimg[PPT];
+  int64_t pix[PPT];
+  float xf[PPT], yf[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    const int i = threadIdx.x + q * NT;
+    const int px = tx * TS + (i % TS);
+    const int py = ty * TS + (i / TS);
+    const int64_t lp = (int64_t)py * jb.W + px;
+    inimg[q] = (px < jb.W) && (py < jb.H) && (lp < jb.npix);
+    pix[q] = jb.pix_base + (inimg[q] ? lp : 0);
+    if (jb.ray_xy) {
+      xf[q] = inimg[q] ? jb.ray_xy[2 * lp] : 0.0f;
+      yf[q] = inimg[q] ? jb.ray_xy[2 * lp + 1] : 0.0f;
+    } else {
+      xf[q] = (float)px - jb.cx;
+      yf[q] = (float)py - jb.cy;
+    }
   }
   const float b3 = jb.beta3;
   const float nb2l = -jb.beta2 * kLog2e;
@@ -782,19 +812,24 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   const float kap = r2 ? nb2l / b3 : 0.0f;
   const bool masked = (PHASE == 0) || !A.tile_np || A.tile_np[2 * gt + (PHASE == 2 ? 1 : 0)] != 0;
 
-  float M, S, Z, E, Zall;
-  if (PHASE == 2) {
-    M = A.st[pix];
-    S = A.st[A.total_pix + pix];
-    Z = A.st[2 * A.total_pix + pix];
-    E = A.st[3 * A.total_pix + pix];
-    Zall = A.st[4 * A.total_pix + pix];
-  } else {
-    M = -3.402823466e38f; S = 0.f; Z = 0.f; E = 0.f; Zall = 0.f;
+  float M[PPT], S[PPT], Z[PPT], E[PPT], Zall[PPT], Mt[PPT];
+  bool nanf[PPT];
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    if (PHASE == 2) {
+      M[q] = A.st[pix[q]];
+      S[q] = A.st[A.total_pix + pix[q]];
+      Z[q] = A.st[2 * A.total_pix + pix[q]];
+      E[q] = A.st[3 * A.total_pix + pix[q]];
+      Zall[q] = A.st[4 * A.total_pix + pix[q]];
+      // a NaN logit seen by the phase-1 pass (kept in the sixth state plane between the passes)
+      nanf[q] = A.st[5 * A.total_pix + pix[q]] != 0.0f;
+    } else {
+      M[q] = -3.402823466e38f; S[q] = 0.f; Z[q] = 0.f; E[q] = 0.f; Zall[q] = 0.f;
+      nanf[q] = false;
+    }
+    Mt[q] = (r2 && PHASE == 2) ? M[q] / b3 : -3.402823466e38f;
   }
-  // a NaN logit seen by the phase-1 pass (kept in the sixth state plane between the passes)
-  bool nanf = (PHASE == 2) ? (A.st[5 * A.total_pix + pix] != 0.0f) : false;
-  float Mt = (r2 && PHASE == 2) ? M / b3 : -3.402823466e38f;
 
   int64_t start, end;
   if (PHASE == 0) { start = 0; end = jb.n; }
@@ -822,8 +857,8 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
         lds[i] = v;
       }
       __syncthreads();
-      if (masked) blend_batch_rec2<SUB, true>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-      else blend_batch_rec2<SUB, false>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+      if (masked) blend_batch_rec2<SUB, true, PPT>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+      else blend_batch_rec2<SUB, false, PPT>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
       __syncthreads();
       continue;
     }
@@ -842,29 +877,35 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
       lds[i] = v;
     }
     __syncthreads();
-    if (safe) blend_batch<SUB, true>(lds, nbr, xf, yf, b3, nb2l, kd, M, S, Z, E, Zall, nanf);
-    else blend_batch<SUB, false>(lds, nbr, xf, yf, b3, nb2l, kd, M, S, Z, E, Zall, nanf);
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      if (safe) blend_batch<SUB, true>(lds, nbr, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
+      else blend_batch<SUB, false>(lds, nbr, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
+    }
     __syncthreads();
   }
 
+  const unsigned long long ev = (unsigned long long)(end - start) *
+                                (unsigned long long)tile_valid(jb, tile, tx, ty, TS);
   if (PHASE == 1) {
-    if (inimg) {
-      A.st[pix] = M;
-      A.st[A.total_pix + pix] = S;
-      A.st[2 * A.total_pix + pix] = Z;
-      A.st[3 * A.total_pix + pix] = E;
-      A.st[4 * A.total_pix + pix] = Zall;
-      A.st[5 * A.total_pix + pix] = nanf ? 1.0f : 0.0f;
-    }
     // exact per-tile minimum of log2 sum exp of the unmasked listed logits; a pixel whose
     // phase-1 listed logits are NaN gets -inf (its tile then lists everything in phase 2)
-    float lm = (S > 0.0f && !nanf) ? M + __log2f(S) : -INFINITY;
-    if (!inimg) lm = INFINITY;
+    float lm = INFINITY;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      if (inimg[q]) {
+        A.st[pix[q]] = M[q];
+        A.st[A.total_pix + pix[q]] = S[q];
+        A.st[2 * A.total_pix + pix[q]] = Z[q];
+        A.st[3 * A.total_pix + pix[q]] = E[q];
+        A.st[4 * A.total_pix + pix[q]] = Zall[q];
+        A.st[5 * A.total_pix + pix[q]] = nanf[q] ? 1.0f : 0.0f;
+        lm = fminf(lm, (S[q] > 0.0f && !nanf[q]) ? M[q] + __log2f(S[q]) : -INFINITY);
+      }
+    }
     lm = block_min<NT>(lm, sh_red);
     if (threadIdx.x == 0) {
       A.tile_lmin[gt] = lm;
-      const unsigned long long ev = (unsigned long long)(end - start) *
-                                    (unsigned long long)tile_valid(jb, tile, tx, ty, TS);
       atomicAdd(&A.cnt->evals, ev);
       atomicAdd(&A.cnt->evals1, ev);
     }
@@ -875,41 +916,45 @@ __global__ __launch_bounds__(TS* TS) void k_blend(BlendArgs A) {
   float ncnt;
   if (PHASE == 0) ncnt = (float)jb.n;
   else ncnt = (float)((A.off1[gt + 1] - A.off1[gt]) + (A.off2[gt + 1] - A.off2[gt]));
-  float D;
-  if (ncnt == 0.0f) D = 0.0f;
-  else if (nanf) D = Zall / ncnt;
-  else D = (Z + 1e-20f * Zall) / (S + 1e-20f * ncnt);
-  const float alpha = -expm1f(-E);
-  if (inimg) {
-    jb.depth[pix - jb.pix_base] = D;
-    jb.alpha[pix - jb.pix_base] = alpha;
-    if (A.save) {
-      A.st[pix] = M;
-      A.st[A.total_pix + pix] = S;
-      A.st[2 * A.total_pix + pix] = Z;
-      A.st[3 * A.total_pix + pix] = E;
-      A.st[4 * A.total_pix + pix] = Zall;
-      A.st[5 * A.total_pix + pix] = D;
+  bool fl = false;
+#pragma unroll
+  for (int q = 0; q < PPT; ++q) {
+    float D;
+    if (ncnt == 0.0f) D = 0.0f;
+    else if (nanf[q]) D = Zall[q] / ncnt;
+    else D = (Z[q] + 1e-20f * Zall[q]) / (S[q] + 1e-20f * ncnt);
+    const float alpha = -expm1f(-E[q]);
+    if (inimg[q]) {
+      jb.depth[pix[q] - jb.pix_base] = D;
+      jb.alpha[pix[q] - jb.pix_base] = alpha;
+      if (A.save) {
+        A.st[pix[q]] = M[q];
+        A.st[A.total_pix + pix[q]] = S[q];
+        A.st[2 * A.total_pix + pix[q]] = Z[q];
+        A.st[3 * A.total_pix + pix[q]] = E[q];
+        A.st[4 * A.total_pix + pix[q]] = Zall[q];
+        A.st[5 * A.total_pix + pix[q]] = D;
+      }
+    }
+    if (PHASE == 2) {
+      // weight floor: 1e-20 * n relative to S can only matter if a dropped pair that may lie
+      // behind the camera pushes the true max logit far above M
+      const float ubm = A.tile_ubm[gt];
+      const float lhs = -43.185f + __log2f((float)max(jb.n, 1)) + fmaxf(0.0f, ubm - M[q]);
+      // (a view with degenerate records lists every pair, and so does a tile completed
+      // densely: nothing was dropped there and the floor term is exact)
+      fl = fl || (inimg[q] && !safe && ncnt < (float)jb.n &&
+                  !(lhs < ((S[q] > 0.0f) ? __log2f(S[q]) : -INFINITY)));
     }
   }
   if (PHASE == 2) {
-    // weight floor: 1e-20 * n relative to S can only matter if a dropped pair that may lie
-    // behind the camera pushes the true max logit far above M
-    const float ubm = A.tile_ubm[gt];
-    const float lhs = -43.185f + __log2f((float)max(jb.n, 1)) + fmaxf(0.0f, ubm - M);
-    // (a view with degenerate records lists every pair, and so does a tile completed densely:
-    // nothing was dropped there and the floor term is exact)
-    const bool fl = inimg && !safe && ncnt < (float)jb.n && !(lhs < ((S > 0.0f) ? __log2f(S) : -INFINITY));
     const int any = __syncthreads_or(fl ? 1 : 0);
     if (threadIdx.x == 0) {
       if (any) atomicAdd(&A.cnt->floor_unresolved, 1ull);
-      atomicAdd(&A.cnt->evals, (unsigned long long)(end - start) *
-                                   (unsigned long long)tile_valid(jb, tile, tx, ty, TS));
+      atomicAdd(&A.cnt->evals, ev);
     }
   } else {
     if (threadIdx.x == 0) {
-      const unsigned long long ev = (unsigned long long)(end - start) *
-                                    (unsigned long long)tile_valid(jb, tile, tx, ty, TS);
       atomicAdd(&A.cnt->evals, ev);
       atomicAdd(&A.cnt->evals1, ev);
     }
@@ -996,7 +1041,7 @@ hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t
 
 template <int TS, int GB>
 static hipError_t launch_blend_tg(hipStream_t s, int phase, int64_t total_tiles, const BlendArgs& a) {
-  const dim3 grid((unsigned)total_tiles), block(TS * TS);
+  const dim3 grid((unsigned)total_tiles), block(BlendShape<TS>::NT);
   if (phase == 0) hipLaunchKernelGGL((k_blend<TS, GB, 0>), grid, block, 0, s, a);
   else if (phase == 1) hipLaunchKernelGGL((k_blend<TS, GB, 1>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((k_blend<TS, GB, 2>), grid, block, 0, s, a);
