@@ -194,6 +194,13 @@ def workload_serving(fea, sc, dev, rates, n_per_rate):
     out["saturated_fwd_bwd_480x640_n4096"] = {"batched_rps": sat_b["throughput_rps"],
                                               "plain_rps": sat_p["throughput_rps"],
                                               "ratio": round(sat_b["throughput_rps"] / sat_p["throughput_rps"], 3)}
+    # the request mix itself with every request queued at once (throughput-bound)
+    mix_b = served(fea, sc, dev, rate=1.0, n=2000, burst=True, max_batch=256)
+    mix_p = served(fea, sc, dev, rate=1.0, n=400, burst=True, max_batch=1, slots=1)
+    out["saturated_mix"] = {"batched_rps": mix_b["throughput_rps"], "batched_p50_ms": mix_b["p50_ms"],
+                            "batched_p99_ms": mix_b["p99_ms"], "mean_batch": mix_b["mean_batch"],
+                            "plain_rps": mix_p["throughput_rps"],
+                            "ratio": round(mix_b["throughput_rps"] / mix_p["throughput_rps"], 3)}
     out["value"] = out["rates"][str(rates[-1])]["p99_ms"]
     out["higher_is_better"] = False
     return out
