@@ -218,7 +218,8 @@ class Renderer:
 
     def debug_state(self) -> dict:
         """Host copies of the last forward's workspace internals (tests only): per-Gaussian
-        records, tile ranges and lists of both phases, per-tile phase-1 thresholds."""
+        records, tile ranges and lists of both phases, per-tile phase-1 thresholds and floor
+        seeds."""
         torch.cuda.synchronize(self.device)
         ws = self._ws.cpu().numpy()
         h = ws[:8 * 12].view(np.uint64)
@@ -229,7 +230,8 @@ class Renderer:
             off2 = ws[o_off2:o_off2 + 8 * (T + 1)].view(np.uint64).astype(np.int64)
             total = int(off2[-1])
             out.update(off1=off1, off2=off2, list=ws[o_list:o_list + 4 * total].view(np.uint32).copy(),
-                       tile_lmin=ws[o_lmin:o_lmin + 4 * T].view(np.float32).copy())
+                       tile_lmin=ws[o_lmin:o_lmin + 4 * T].view(np.float32).copy(),
+                       tile_seed=ws[o_lmin + 4 * T:o_lmin + 8 * T].view(np.float32).copy())
         return out
 
     # -- backward -----------------------------------------------------------------------

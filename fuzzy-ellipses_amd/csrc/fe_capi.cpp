@@ -195,7 +195,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->o_flags = p; p = align256(p + sizeof(int32_t) * n_views);
   L->o_rec = p; p = align256(p + sizeof(float) * kRecFloats * (size_t)std::max<int64_t>(rec, 1));
   L->o_rec2 = p; p = align256(p + sizeof(float) * kRec2Floats * (size_t)std::max<int64_t>(rec, 1));
-  L->o_lmin = p; p = align256(p + sizeof(float) * tiles);
+  L->o_lmin = p; p = align256(p + sizeof(float) * 2 * tiles);   // thresholds, then floor seeds
   L->o_ubm = p; p = align256(p + sizeof(float) * tiles);
   L->o_state = p; p = align256(p + sizeof(float) * 6 * pix);
   if (o.mode == FE_MODE_CULLED) {
@@ -346,6 +346,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   float* rec2 = at<float>(ws, L.o_rec2);
   float* state = at<float>(ws, L.o_state);
   float* lmin = at<float>(ws, L.o_lmin);
+  float* seed = lmin + L.total_tiles;   // per-tile floor seed, after the thresholds
   float* ubm = at<float>(ws, L.o_ubm);
   FE_HIP(launch_init(s, cnt, flags, L.n_jobs));
   bool any_pose = false;
@@ -356,7 +357,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
       return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, rec2, nullptr, nullptr, nullptr,
-                          nullptr, state, L.total_pix, lmin, ubm, cnt, flags, L.save);
+                          nullptr, state, L.total_pix, lmin, seed, ubm, cnt, flags, L.save);
     }));
     return FE_OK;
   }
@@ -370,18 +371,18 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   uint32_t* tnp = at<uint32_t>(ws, L.o_tnp);
   // phase 1
   FE_HIP(timed(s, FE_K_MASK1, [&] {
-    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, tnp,
+    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, seed, ubm, tnp,
                             flags, cnt);
   }));
   FE_HIP(timed(s, FE_K_SCAN1, [&] { return launch_scan(s, count, L.total_tiles, partial, off1, nullptr); }));
   FE_HIP(timed(s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }));
   FE_HIP(timed(s, FE_K_BLEND1, [&] {
     return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
-                        L.total_pix, lmin, ubm, cnt, flags, L.save);
+                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save);
   }));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
-    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, ubm, tnp,
+    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, seed, ubm, tnp,
                             flags, cnt);
   }));
   FE_HIP(timed(s, FE_K_SCAN2, [&] {
@@ -390,7 +391,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   FE_HIP(timed(s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }));
   FE_HIP(timed(s, FE_K_BLEND2, [&] {
     return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
-                        L.total_pix, lmin, ubm, cnt, flags, L.save);
+                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save);
   }));
   return FE_OK;
 }

@@ -198,9 +198,17 @@ __device__ __forceinline__ float4 ray_rect(float xmin, float xmax, float ymin, f
 // centre ray to any tile ray is at most the largest corner angle, bounded by its tangent.
 //   front        : like plus, with a 2^-10 relative margin on sd_hi < |q| (used only to let
 //                  the blend skip the t* > 0 test on a tile, never for membership)
+//   can_mask, d0m_lo, negt_hi : where a tile ray can have t* <= 0, the angle theta between it
+//                  and q is at least 90 deg; with theta_c the centre ray's angle and phi the
+//                  tile's angular radius, theta <= theta_c + phi there, so
+//                  d0 = |q|^2 sin^2(theta) >= d0m_lo and -t* <= negt_hi on the masked rays
+//                  (can_mask false: no tile ray has t* <= 0).  This bounds the logit of a
+//                  masked pair far below the one from (sd_lo, t_any_lb), whose extremes
+//                  cannot occur at the same ray.
 struct PairBound {
   float sd_lo, sd_hi, t_lb, t_any_lb, t_ub;
-  bool plus, unbounded, front;
+  float d0m_lo, negt_hi;
+  bool plus, unbounded, front, can_mask;
 };
 
 __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& tg) {
@@ -250,9 +258,29 @@ __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& t
   pb.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-pb.sd_lo, pb.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
   // upper bound of t* on the tile: q.u/|u| <= min(|q|, q.u_c/|u_c| + |q| tan(angle)), |u| >= |u_c| - rho
   pb.t_ub = (un > rho) ? fminf(qn, qu / un + dth) / (un - rho) : INFINITY;
+  // masked part of the tile cone: cos/sin of theta_c + phi from cos/sin of each
+  const float cth = qu / (qn * un);
+  const float sth = sdc / qn;
+  const float cph = 1.0f / sqrtf(fmaf(tanm, tanm, 1.0f));
+  const float sph = tanm * cph;
+  const float cmax = fmaf(cth, cph, -(sth * sph));
+  const float smax = fmaf(sth, cph, cth * sph);
+  pb.can_mask = !(cmax > 0.0f);
+  // past 180 deg (smax < 0) the cone holds the ray straight away from q: sin^2 >= 0, -cos <= 1
+  pb.d0m_lo = (smax > 0.0f) ? qq * (smax * smax) : 0.0f;
+  pb.negt_hi = (un > rho) ? qn * ((smax > 0.0f) ? fminf(-cmax, 1.0f) : 1.0f) / (un - rho) : INFINITY;
   if (!(pb.sd_lo == pb.sd_lo)) pb.unbounded = true;  // NaN anywhere -> keep
   pb.front = pb.front && !pb.unbounded;
   return pb;
+}
+
+// Upper bound (log2 units) of the depth logit of a pair on the tile rays where it is masked
+// (t* <= 0), or -inf where it cannot be masked: the smaller of two bounds.
+__device__ __forceinline__ float masked_logit_ub(const PairBound& pb, float w, float b3l, float b2l) {
+  const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
+  const float u1 = fmaf(b3l, dlo, -(b2l * pb.t_any_lb));
+  const float u2 = pb.can_mask ? fmaf(b3l, fmaf(-0.5f, pb.d0m_lo, w), b2l * pb.negt_hi) : -INFINITY;
+  return fminf(u1, u2);
 }
 
 }  // namespace fe

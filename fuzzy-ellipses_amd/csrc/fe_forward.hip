@@ -27,19 +27,22 @@ __device__ __forceinline__ float nan_to_num(float t) {
   return (t == t) ? fminf(fmaxf(t, -3.402823466e38f), 3.402823466e38f) : 0.0f;
 }
 
-template <int NT>
-__device__ __forceinline__ float block_min(float v, float* sh) {
-  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+// block-wide min (MAX = false) or max (MAX = true); every thread gets the result
+template <int NT, bool MAX = false>
+__device__ __forceinline__ float block_ext(float v, float* sh) {
+  auto op = [](float a, float b) { return MAX ? fmaxf(a, b) : fminf(a, b); };
+  for (int o = 32; o > 0; o >>= 1) v = op(v, __shfl_xor(v, o, 64));
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (NT > 64) {
     if (lane == 0) sh[wave] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
-      for (int w = 1; w < NT / 64; ++w) v = fminf(v, sh[w]);
+      for (int w = 1; w < NT / 64; ++w) v = op(v, sh[w]);
       sh[0] = v;
     }
     __syncthreads();
     v = sh[0];
+    __syncthreads();   // sh is reused by the next reduction
   }
   return v;
 }
@@ -178,9 +181,11 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
 //   writing mask1.
 // PHASE 2 keeps pairs absent from mask1 whose depth-logit bound reaches the tile threshold
 //   tau = min_pixels(log2-sum-exp of phase-1 logits) - delta2, writing mask2.  It also takes
-//   the largest logit bound `ubm` of the dropped pairs that might lie behind the camera: if
-//   such a pair could raise the max logit so far that the 1e-20 weight floor becomes visible
-//   (ubm - lmin >= floor_thr), the tile is completed densely (every remaining pair).
+//   the largest bound `ubm` of the logit, where masked, of every pair absent from mask1 that
+//   might lie behind the camera, started from the phase-1 floor seed: if the max logit could
+//   rise so far above the log2-sum-exp that the 1e-20 weight floor becomes visible
+//   (ubm - lmin >= floor_thr, or a seed of +inf), the tile is completed densely (every
+//   remaining pair), since the floor term sums over all n Gaussians.
 // ------------------------------------------------------------------------------------------
 constexpr int kWinWords = 64;   // membership words per LDS window
 
@@ -191,6 +196,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
                                                    unsigned long long* __restrict__ mask2,
                                                    uint32_t* __restrict__ count,
                                                    const float* __restrict__ tile_lmin,
+                                                   const float* __restrict__ tile_seed,
                                                    float* __restrict__ tile_ubm,
                                                    uint32_t* __restrict__ tile_np,
                                                    const int32_t* __restrict__ job_flags,
@@ -322,18 +328,19 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
           const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
           keep_s = pb.unbounded || (ub >= tau_s);
           plus_s = pb.plus;
-          if (!pb.plus) ubm_s = fmaf(b3l, dlo, -(b2l * pb.t_any_lb));
+          if (!pb.plus) ubm_s = masked_logit_ub(pb, w, b3l, b2l);
         }
         keep1 = keep_s;
       }
       if (PHASE == 2 && valid && !forced) {
-        // pairs listed in phase 1 are skipped; a dropped pair that may lie behind the camera
-        // raises the tile's ubm (here from the group bound, which covers every sub-tile)
+        // pairs listed in phase 1 are skipped; any other pair that may lie behind the camera
+        // raises the tile's ubm: here from the group bound (which covers every sub-tile) when
+        // the group drops it or the group is the tile, else from the tile bound in pass 2
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
           if (k < nsub) {
             const bool listed = (wlist[k][c] >> lane) & 1ull;
-            if (!listed && !keep_s && !plus_s) ubm[k] = fmaxf(ubm[k], ubm_s);
+            if (!listed && !plus_s && (single || !keep_s)) ubm[k] = fmaxf(ubm[k], ubm_s);
           }
         }
       }
@@ -401,7 +408,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
             const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
             const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
             keep = pb.unbounded || (ub >= tau[k]);
-            if (!keep && !pb.plus) ubm[k] = fmaxf(ubm[k], fmaf(b3l, dlo, -(b2l * pb.t_any_lb)));
+            if (!pb.plus) ubm[k] = fmaxf(ubm[k], masked_logit_ub(pb, w, b3l, b2l));
           }
           if (keep) {
             atomicOr(&wmask[k][c], 1ull << bit);
@@ -444,7 +451,8 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
       uint32_t total = tcnt[k];
       uint32_t np = tnp[k];
       if (PHASE == 2) {
-        const float ubm_t = fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k]));
+        const float ubm_t = fmaxf(fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k])),
+                                  tile_seed[gt]);
         const bool dense_tile = (ubm_t - lmin[k] >= jb.floor_thr);   // false when nothing in doubt
         if (dense_tile) {
           // complete the tile: every pair not in phase 1
@@ -755,6 +763,7 @@ struct BlendArgs {
   float* st;            // per-pixel state, 6 planes of total_pix: M S Z E Zall D
   int64_t total_pix;
   float* tile_lmin;
+  float* tile_seed;     // phase 1 writes the floor seed of each tile (phase-2 membership input)
   const float* tile_ubm;
   Counters* cnt;
   const int32_t* job_flags;
@@ -890,7 +899,11 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
   if (PHASE == 1) {
     // exact per-tile minimum of log2 sum exp of the unmasked listed logits; a pixel whose
     // phase-1 listed logits are NaN gets -inf (its tile then lists everything in phase 2)
-    float lm = INFINITY;
+    // The floor seed: the 1e-20 weight floor is already visible at a pixel whose listed max
+    // logit M (masked pairs included) lies floor_thr above its log2-sum-exp M + log2 S, and
+    // at a pixel with S = 0 any logit above the tile's threshold makes it so; the phase-2
+    // membership completes such a tile densely (seed +inf), or starts its bound from M.
+    float lm = INFINITY, gap = -INFINITY, m0 = -INFINITY;
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
       if (inimg[q]) {
@@ -900,12 +913,20 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
         A.st[3 * A.total_pix + pix[q]] = E[q];
         A.st[4 * A.total_pix + pix[q]] = Zall[q];
         A.st[5 * A.total_pix + pix[q]] = nanf[q] ? 1.0f : 0.0f;
-        lm = fminf(lm, (S[q] > 0.0f && !nanf[q]) ? M[q] + __log2f(S[q]) : -INFINITY);
+        const bool pos = S[q] > 0.0f && !nanf[q];
+        const float l2s = pos ? __log2f(S[q]) : 0.0f;
+        lm = fminf(lm, pos ? M[q] + l2s : -INFINITY);
+        if (nanf[q]) gap = INFINITY;
+        else if (pos) gap = fmaxf(gap, -l2s);
+        else m0 = fmaxf(m0, M[q]);
       }
     }
-    lm = block_min<NT>(lm, sh_red);
+    lm = block_ext<NT>(lm, sh_red);
+    gap = block_ext<NT, true>(gap, sh_red);
+    m0 = block_ext<NT, true>(m0, sh_red);
     if (threadIdx.x == 0) {
       A.tile_lmin[gt] = lm;
+      A.tile_seed[gt] = (gap >= jb.floor_thr) ? INFINITY : m0;
       atomicAdd(&A.cnt->evals, ev);
       atomicAdd(&A.cnt->evals1, ev);
     }
@@ -998,24 +1019,24 @@ hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max
 
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
                             int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
-                            uint32_t* count, const float* tile_lmin, float* tile_ubm, uint32_t* tile_np,
-                            const int32_t* job_flags, Counters* cnt) {
+                            uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
+                            uint32_t* tile_np, const int32_t* job_flags, Counters* cnt) {
   // the group factor is 4 for 8-pixel tiles, 2 for 16, 1 for 32 and for ray lists
   const dim3 g((unsigned)total_stiles), b(256);
   if (phase == 1) {
     if (ts == 8) hipLaunchKernelGGL((k_tile_mask<1, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                                    tile_ubm, tile_np, job_flags, cnt);
+                                    tile_seed, tile_ubm, tile_np, job_flags, cnt);
     else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<1, 2>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
-                                          tile_lmin, tile_ubm, tile_np, job_flags, cnt);
+                                          tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt);
     else hipLaunchKernelGGL((k_tile_mask<1, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                            tile_ubm, tile_np, job_flags, cnt);
+                            tile_seed, tile_ubm, tile_np, job_flags, cnt);
   } else {
     if (ts == 8) hipLaunchKernelGGL((k_tile_mask<2, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                                    tile_ubm, tile_np, job_flags, cnt);
+                                    tile_seed, tile_ubm, tile_np, job_flags, cnt);
     else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 2>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
-                                          tile_lmin, tile_ubm, tile_np, job_flags, cnt);
+                                          tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt);
     else hipLaunchKernelGGL((k_tile_mask<2, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                            tile_ubm, tile_np, job_flags, cnt);
+                            tile_seed, tile_ubm, tile_np, job_flags, cnt);
   }
   return hipGetLastError();
 }
@@ -1063,10 +1084,10 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
                         int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                         const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
-                        float* tile_lmin, const float* tile_ubm, Counters* cnt, const int32_t* job_flags,
-                        int save) {
-  BlendArgs a{jobs, n_jobs, rec, rec2, tile_np, off1, off2, list, st, total_pix, tile_lmin, tile_ubm, cnt, job_flags,
-              save};
+                        float* tile_lmin, float* tile_seed, const float* tile_ubm, Counters* cnt,
+                        const int32_t* job_flags, int save) {
+  BlendArgs a{jobs, n_jobs, rec, rec2, tile_np, off1, off2, list, st, total_pix, tile_lmin, tile_seed, tile_ubm, cnt,
+              job_flags, save};
   switch (ts) {
     case 8: return launch_blend_t<8>(s, gb, phase, total_tiles, a);
     case 16: return launch_blend_t<16>(s, gb, phase, total_tiles, a);

@@ -22,8 +22,8 @@ hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max
                              int32_t* job_flags);
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
                             int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
-                            uint32_t* count, const float* tile_lmin, float* tile_ubm, uint32_t* tile_np,
-                            const int32_t* job_flags, Counters* cnt);
+                            uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
+                            uint32_t* tile_np, const int32_t* job_flags, Counters* cnt);
 int64_t scan_partial_count(int64_t n);
 hipError_t launch_scan(hipStream_t s, const uint32_t* cnt, int64_t n, unsigned long long* partial,
                        unsigned long long* off, const unsigned long long* base_src);
@@ -34,8 +34,8 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
                         int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                         const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
-                        float* tile_lmin, const float* tile_ubm, Counters* cnt, const int32_t* job_flags,
-                        int save);
+                        float* tile_lmin, float* tile_seed, const float* tile_ubm, Counters* cnt,
+                        const int32_t* job_flags, int save);
 
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
                             int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,

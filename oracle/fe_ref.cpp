@@ -223,7 +223,8 @@ void fe_ref_records(int n, const float* means, const float* prec9, const float* 
 
 struct Bound {
   float sd_lo, sd_hi, t_lb, t_any_lb, t_ub;
-  bool plus, unbounded;
+  float d0m_lo, negt_hi;
+  bool plus, unbounded, can_mask;
 };
 
 static Bound bound_of(const float* r, float Xc, float Yc, float hx, float hy) {
@@ -267,12 +268,32 @@ static Bound bound_of(const float* r, float Xc, float Yc, float hx, float hy) {
   o.t_lb = o.plus ? sqrtf(fmaxf(fmaf(-o.sd_hi, o.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
   o.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-o.sd_lo, o.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
   o.t_ub = (un > rho) ? fminf(qn, qu / un + dth) / (un - rho) : INFINITY;
+  // masked part of the tile cone (theta >= 90 deg, theta <= theta_c + phi)
+  const float cth = qu / (qn * un);
+  const float sth = sdc / qn;
+  const float cph = 1.0f / sqrtf(fmaf(tanm, tanm, 1.0f));
+  const float sph = tanm * cph;
+  const float cmax = fmaf(cth, cph, -(sth * sph));
+  const float smax = fmaf(sth, cph, cth * sph);
+  o.can_mask = !(cmax > 0.0f);
+  o.d0m_lo = (smax > 0.0f) ? qq * (smax * smax) : 0.0f;
+  o.negt_hi = (un > rho) ? qn * ((smax > 0.0f) ? fminf(-cmax, 1.0f) : 1.0f) / (un - rho) : INFINITY;
   if (!(o.sd_lo == o.sd_lo)) o.unbounded = true;
   return o;
 }
 
+// Upper bound (log2 units) of the depth logit of a pair on the tile rays where it is masked.
+static float masked_logit_ub(const Bound& o, float w, float b3l, float b2l) {
+  const float dlo = fmaf(-0.5f * o.sd_lo, o.sd_lo, w);
+  const float u1 = fmaf(b3l, dlo, -(b2l * o.t_any_lb));
+  const float u2 = o.can_mask ? fmaf(b3l, fmaf(-0.5f, o.d0m_lo, w), b2l * o.negt_hi) : -INFINITY;
+  return fminf(u1, u2);
+}
+
 // Tile lists of one view.  phase 1: alpha bound.  phase 2 (tile_lmin != NULL): pairs not in
-// phase 1 whose depth-logit bound reaches tile_lmin[t] - delta2.  Membership is pre-tested
+// phase 1 whose depth-logit bound reaches tile_lmin[t] - delta2; a tile is completed with every
+// pair not in phase 1 when the largest masked-logit bound of those pairs, or the phase-1 floor
+// seed tile_seed[t] (NULL: none), reaches tile_lmin[t] + floor_thr.  Membership is pre-tested
 // on groups of sf x sf tiles (sf = 4, 2, 1 for ts = 8, 16, 32): a pair the group bound
 // rejects is rejected for every tile of the group.  Writes counts[T] and the concatenated,
 // per-tile ascending Gaussian ids to `list` (capacity n*T); returns the total.
@@ -298,7 +319,7 @@ static Geo geo_of(int tx, int ty, int ts, int W, int H, float cx, float cy) {
 template <typename GeoT, typename GroupGeoT>
 static int64_t tiles_impl(int n, const float* rec, int any_degenerate, int tx, int ty, int sf, GeoT geo,
                           GroupGeoT ggeo, float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
-                          const float* tile_lmin, uint32_t* counts, uint32_t* list) {
+                          const float* tile_lmin, const float* tile_seed, uint32_t* counts, uint32_t* list) {
   const float b3l = beta3 * kLog2e, b2l = beta2 * kLog2e;
   int64_t pos = 0;
   std::vector<char> in1(n > 0 ? n : 1, 0);
@@ -341,7 +362,9 @@ static int64_t tiles_impl(int n, const float* rec, int any_degenerate, int tx, i
     }
     uint32_t c = 0;
     const int64_t start = pos;
-    float ubm = -INFINITY;
+    // largest logit bound, where masked, of the pairs absent from phase 1, started from the
+    // phase-1 floor seed of the tile (+inf when the floor is already visible)
+    float ubm = (tile_lmin && tile_seed) ? tile_seed[tile] : -INFINITY;
     for (int j = 0; j < n; ++j) {
       const float* r = rec + 16 * (size_t)j;
       int32_t fl;
@@ -369,9 +392,9 @@ static int64_t tiles_impl(int n, const float* rec, int any_degenerate, int tx, i
             const float dlo = fmaf(-0.5f * o.sd_lo, o.sd_lo, w);
             const float ub = fmaf(b3l, dlo, -(b2l * o.t_lb));
             keep2 = o.unbounded || (ub >= tau);
-            if (!keep2 && !o.plus) ubm = fmaxf(ubm, fmaf(b3l, dlo, -(b2l * o.t_any_lb)));
+            if (!o.plus) ubm = fmaxf(ubm, masked_logit_ub(o, w, b3l, b2l));
           } else if (!og.plus) {
-            ubm = fmaxf(ubm, fmaf(b3l, dlo_g, -(b2l * og.t_any_lb)));
+            ubm = fmaxf(ubm, masked_logit_ub(og, w, b3l, b2l));
           }
         }
       }
@@ -401,18 +424,65 @@ extern "C" {
 
 int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, int ts, float cx, float cy,
                      float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
-                     const float* tile_lmin, uint32_t* counts, uint32_t* list) {
+                     const float* tile_lmin, const float* tile_seed, uint32_t* counts, uint32_t* list) {
   const int tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
   const int sf = (ts == 8) ? 4 : (ts == 16 ? 2 : 1);
   return tiles_impl(
       n, rec, any_degenerate, tx, ty, sf, [&](int a, int b) { return geo_of(a, b, ts, W, H, cx, cy); },
       [&](int a, int b) { return geo_of(a, b, ts * sf, W, H, cx, cy); }, beta2, beta3, c_alpha, delta2, floor_thr,
-      tile_lmin, counts, list);
+      tile_lmin, tile_seed, counts, list);
 }
 
 // Ray-list views: tile t holds rays [t*ts*ts, (t+1)*ts*ts) of (x', y') camera-plane
 // coordinates; its rectangle is the rays' bounding box widened by 2^-20 of the largest
 // coordinate (the GPU's ray_rect, same float operations).
+// Check of the masked-logit bound: for every (tile, Gaussian) of a pixel-grid view, the exact
+// (float64) logit b3l*d2 - b2l*t* at every pixel of the tile where t* <= 0 must not exceed
+// masked_logit_ub of the tile's bound (with a relative slack of 1e-5 for the bound's own
+// float32 rounding).  Returns the number of violating (tile, Gaussian) pairs; *checked gets
+// the number of (pixel, Gaussian) masked evaluations compared.
+int64_t fe_ref_masked_bound_check(int n, const float* rec, int H, int W, int ts, float cx, float cy, float beta2,
+                                  float beta3, int64_t* checked) {
+  const float b3l = beta3 * kLog2e, b2l = beta2 * kLog2e;
+  const int tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
+  int64_t bad = 0, cnt = 0;
+#pragma omp parallel for reduction(+ : bad, cnt) schedule(dynamic)
+  for (int tile = 0; tile < tx * ty; ++tile) {
+    const int txi = tile % tx, tyi = tile / tx;
+    const Geo g = geo_of(txi, tyi, ts, W, H, cx, cy);
+    for (int j = 0; j < n; ++j) {
+      const float* r = rec + 16 * (size_t)j;
+      const Bound o = bound_of(r, g.Xc, g.Yc, g.hx, g.hy);
+      if (o.unbounded) continue;
+      const float ub = masked_logit_ub(o, r[12], b3l, b2l);
+      double worst = -INFINITY;
+      for (int y = tyi * ts; y < std::min(H, (tyi + 1) * ts); ++y)
+        for (int x = txi * ts; x < std::min(W, (txi + 1) * ts); ++x) {
+          const double xf = (double)x - cx, yf = (double)y - cy;
+          double u[3], qu = 0.0, uu = 0.0;
+          for (int i = 0; i < 3; ++i) {
+            u[i] = (double)r[i] * xf + (double)r[3 + i] * yf + (double)r[6 + i];
+            qu += (double)r[9 + i] * u[i];
+            uu += u[i] * u[i];
+          }
+          const double t = qu / uu;
+          if (!(t <= 0.0)) continue;
+          double d0 = 0.0;
+          for (int i = 0; i < 3; ++i) {
+            const double v = t * u[i] - (double)r[9 + i];
+            d0 += v * v;
+          }
+          const double l = (double)b3l * ((double)r[12] - 0.5 * d0) - (double)b2l * t;
+          worst = std::max(worst, l);
+          ++cnt;
+        }
+      if (worst > (double)ub + 1e-5 * std::fabs((double)ub) + 1e-4) ++bad;
+    }
+  }
+  if (checked) *checked = cnt;
+  return bad;
+}
+
 void fe_ref_ray_rects(int64_t n_rays, int ts, const float* xy, float* rects) {
   const int64_t tt = (int64_t)ts * ts;
   const int64_t T = (n_rays + tt - 1) / tt;
@@ -433,7 +503,7 @@ void fe_ref_ray_rects(int64_t n_rays, int ts, const float* xy, float* rects) {
 
 int64_t fe_ref_tiles_rects(int n, const float* rec, int any_degenerate, int64_t n_tiles, const float* rects,
                            float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
-                           const float* tile_lmin, uint32_t* counts, uint32_t* list) {
+                           const float* tile_lmin, const float* tile_seed, uint32_t* counts, uint32_t* list) {
   auto g = [&](int, int b) {
     Geo o;
     o.Xc = rects[4 * (int64_t)b]; o.Yc = rects[4 * (int64_t)b + 1];
@@ -441,7 +511,7 @@ int64_t fe_ref_tiles_rects(int n, const float* rec, int any_degenerate, int64_t 
     return o;
   };
   return tiles_impl(n, rec, any_degenerate, 1, (int)n_tiles, 1, g, g, beta2, beta3, c_alpha, delta2, floor_thr,
-                    tile_lmin, counts, list);
+                    tile_lmin, tile_seed, counts, list);
 }
 
 }  // extern "C"

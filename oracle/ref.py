@@ -33,6 +33,10 @@ def load():
     if not os.path.exists(LIB_PATH):
         build()
     lib = ctypes.CDLL(LIB_PATH)
+    lib.fe_ref_masked_bound_check.restype = ctypes.c_int64
+    lib.fe_ref_masked_bound_check.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                              ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                              ctypes.c_void_p]
     lib.fe_ref_render_f64.restype = None
     lib.fe_ref_render_f64.argtypes = [ctypes.c_int, _f64p, _f64p, _f64p, _f64p, _f64p, ctypes.c_double,
                                       ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -49,13 +53,13 @@ def load():
     lib.fe_ref_tiles.restype = ctypes.c_int64
     lib.fe_ref_tiles.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                  ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                                 ctypes.c_float, ctypes.c_float, ctypes.c_void_p, _u32p, _u32p]
+                                 ctypes.c_float, ctypes.c_float, ctypes.c_void_p, ctypes.c_void_p, _u32p, _u32p]
     lib.fe_ref_ray_rects.restype = None
     lib.fe_ref_ray_rects.argtypes = [ctypes.c_int64, ctypes.c_int, _f32p, _f32p]
     lib.fe_ref_tiles_rects.restype = ctypes.c_int64
     lib.fe_ref_tiles_rects.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int64, _f32p, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
-                                       ctypes.c_void_p, _u32p, _u32p]
+                                       ctypes.c_void_p, ctypes.c_void_p, _u32p, _u32p]
     _lib = lib
     return lib
 
@@ -123,9 +127,18 @@ def cull_constants(n, alpha_eps=2e-5, depth_eps=1e-6):
             float(np.float32(13.0 * 3.321928094887362 - ln_n * LOG2E)))
 
 
-def tiles(scene, cam, ts, tile_lmin=None):
+def _fptr(a):
+    """(array kept alive, void pointer) of an optional float32 vector."""
+    if a is None:
+        return None, None
+    arr = np.ascontiguousarray(a, np.float32)
+    return arr, arr.ctypes.data_as(ctypes.c_void_p)
+
+
+def tiles(scene, cam, ts, tile_lmin=None, tile_seed=None):
     """Per-tile sorted Gaussian lists: (counts (T,), concatenated ids) for phase 1, or for
-    phase 2 when the per-tile thresholds `tile_lmin` (log2 units) are given."""
+    phase 2 when the per-tile thresholds `tile_lmin` (log2 units) are given, with the phase-1
+    floor seeds `tile_seed` (log2 units, +inf forces dense completion; None: -inf)."""
     lib = load()
     rec, deg = records(scene, cam)
     n = scene.n
@@ -134,15 +147,25 @@ def tiles(scene, cam, ts, tile_lmin=None):
     counts = np.zeros(T, np.uint32)
     lst = np.zeros(max(n * T, 1), np.uint32)
     c_alpha, delta2, floor_thr = cull_constants(n)
-    lm = None
-    if tile_lmin is not None:
-        lm_arr = np.ascontiguousarray(tile_lmin, np.float32)
-        lm = lm_arr.ctypes.data_as(ctypes.c_void_p)
+    lm_arr, lm = _fptr(tile_lmin)
+    sd_arr, sd = _fptr(tile_seed)
     total = lib.fe_ref_tiles(n, np.ascontiguousarray(rec.reshape(-1) if n else np.zeros(16, np.float32)), deg,
                              cam.H, cam.W, ts, np.float32(cam.cx), np.float32(cam.cy), np.float32(scene.beta2),
                              np.float32(scene.beta3), np.float32(c_alpha), np.float32(delta2),
-                             np.float32(floor_thr), lm, counts, lst)
+                             np.float32(floor_thr), lm, sd, counts, lst)
     return counts, lst[:total].copy()
+
+
+def masked_bound_check(scene, cam, ts):
+    """(violating (tile, Gaussian) pairs, masked (pixel, Gaussian) evaluations checked) of the
+    membership's masked-logit bound against exact float64 logits over every pixel."""
+    lib = load()
+    rec, _ = records(scene, cam)
+    checked = ctypes.c_int64(0)
+    bad = lib.fe_ref_masked_bound_check(scene.n, np.ascontiguousarray(rec.reshape(-1)), cam.H, cam.W, ts,
+                                        np.float32(cam.cx), np.float32(cam.cy), np.float32(scene.beta2),
+                                        np.float32(scene.beta3), ctypes.byref(checked))
+    return int(bad), int(checked.value)
 
 
 def ray_rects(xy, ts):
@@ -154,7 +177,7 @@ def ray_rects(xy, ts):
     return out
 
 
-def tiles_rays(scene, R, t, xy, ts, tile_lmin=None):
+def tiles_rays(scene, R, t, xy, ts, tile_lmin=None, tile_seed=None):
     """Membership lists of a ray-list view (records with fx = fy = 1), as `tiles`."""
     from types import SimpleNamespace
     lib = load()
@@ -166,11 +189,9 @@ def tiles_rays(scene, R, t, xy, ts, tile_lmin=None):
     counts = np.zeros(T, np.uint32)
     lst = np.zeros(max(n * T, 1), np.uint32)
     c_alpha, delta2, floor_thr = cull_constants(n)
-    lm = None
-    if tile_lmin is not None:
-        lm_arr = np.ascontiguousarray(tile_lmin, np.float32)
-        lm = lm_arr.ctypes.data_as(ctypes.c_void_p)
+    lm_arr, lm = _fptr(tile_lmin)
+    sd_arr, sd = _fptr(tile_seed)
     total = lib.fe_ref_tiles_rects(n, np.ascontiguousarray(rec.reshape(-1) if n else np.zeros(16, np.float32)), deg,
                                    T, rects.reshape(-1), np.float32(scene.beta2), np.float32(scene.beta3),
-                                   np.float32(c_alpha), np.float32(delta2), np.float32(floor_thr), lm, counts, lst)
+                                   np.float32(c_alpha), np.float32(delta2), np.float32(floor_thr), lm, sd, counts, lst)
     return counts, lst[:total].copy()

@@ -122,7 +122,7 @@ def test_looser_alpha_budget_lists_fewer_pairs(fea, scenes_mod):
         lst = np.zeros(scene.n * T, np.uint32)
         tot = lib.fe_ref_tiles(scene.n, np.ascontiguousarray(rec.reshape(-1)), deg, 64, 64, 16, np.float32(cam.cx),
                                np.float32(cam.cy), np.float32(scene.beta2), np.float32(scene.beta3),
-                               np.float32(c_alpha), np.float32(1.0), np.float32(40.0), None, c, lst)
+                               np.float32(c_alpha), np.float32(1.0), np.float32(40.0), None, None, c, lst)
         counts.append(tot)
     assert counts[0] >= counts[1] >= counts[2]
 

@@ -22,7 +22,7 @@ def _check(fea, scene, cam, ts):
     np.testing.assert_array_equal(np.diff(off1), c1.astype(np.int64))
     np.testing.assert_array_equal(lst[:off1[-1]], l1)
     # phase 2 given the GPU's per-tile thresholds (they come from the phase-1 blend)
-    c2, l2 = ref.tiles(scene, cam, ts, tile_lmin=st["tile_lmin"])
+    c2, l2 = ref.tiles(scene, cam, ts, tile_lmin=st["tile_lmin"], tile_seed=st["tile_seed"])
     np.testing.assert_array_equal(np.diff(off2), c2.astype(np.int64))
     assert off2[0] == off1[-1]
     np.testing.assert_array_equal(lst[off2[0]:off2[-1]], l2)

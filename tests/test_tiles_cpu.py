@@ -89,3 +89,87 @@ def test_degenerate_gaussian_forces_dense(scenes_mod):
     assert deg == 1
     counts, lst = ref.tiles(scene, cam, 16)
     assert np.all(counts == scene.n)
+
+
+def _floor_scene(sc, kind):
+    """Scenes where masked pairs hold large logits: cameras inside a cube of Gaussians and
+    inside a shell of them (every ray meets Gaussians behind the camera), and a wide view."""
+    if kind == "cube_inside":
+        rng = np.random.default_rng(7)
+        m, p, w = sc.random_gaussians(rng, 200)
+        cam = sc.make_camera(np.array([0.12, -0.07, 0.18]), 64, 64, 40.0)
+    elif kind == "shell_centre":
+        rng = np.random.default_rng(8)
+        m, p, w = sc.random_gaussians(rng, 512, shell=True)
+        cam = sc.make_camera(np.array([0.02, -0.03, 0.05]), 64, 64, 40.0)
+    elif kind == "shell_offc":
+        rng = np.random.default_rng(8)
+        m, p, w = sc.random_gaussians(rng, 512, shell=True)
+        cam = sc.make_camera(np.array([0.3, -0.2, 0.4]), 64, 64, 40.0)
+    else:
+        rng = np.random.default_rng(9)
+        m, p, w = sc.random_gaussians(rng, 300)
+        cam = sc.make_camera(3.0 * sc.random_dir(rng), 128, 128, 120.0)
+    return sc.Scene(m, p, w, [cam]), cam
+
+
+@pytest.mark.parametrize("kind", ["cube_inside", "shell_centre", "shell_offc", "wide_fov"])
+@pytest.mark.parametrize("ts", [8, 16])
+def test_dense_completion_covers_the_weight_floor(scenes_mod, kind, ts):
+    """Where phase 2 leaves a tile incomplete, the 1e-20 weight floor (summed over all n
+    Gaussians in the reference's depth) is negligible at every pixel: the max logit over ALL
+    pairs, masked ones included, stays below floor_thr above the log2-sum-exp of the listed
+    unmasked pairs.  Phase-1 inputs (thresholds, floor seeds) are computed in float64 from the
+    phase-1 lists, as the phase-1 blend computes them."""
+    sc = scenes_mod
+    scene, cam = _floor_scene(sc, kind)
+    zs, d2 = _pairs(scene, cam)
+    _, _, floor_thr = ref.cull_constants(scene.n)
+    T = ((cam.W + ts - 1) // ts) * ((cam.H + ts - 1) // ts)
+    tile = _tile_of(cam, ts)
+    counts, lst = ref.tiles(scene, cam, ts)
+    m1, _ = _member(counts, lst, T, scene.n)
+    l2 = LOG2E * (scene.beta3 * d2 - scene.beta2 * zs)
+
+    def lse_of(listed):
+        mx = np.where(listed, l2, -np.inf).max(1)            # max over listed, masked included
+        with np.errstate(invalid="ignore", divide="ignore", over="ignore"):
+            s = np.sum(np.where(listed & (zs > 0), np.exp2(l2 - mx[:, None]), 0.0), 1)
+            return mx, s, mx + np.log2(s)
+
+    mx, s1, lse1 = lse_of(m1[tile])
+    lse1 = np.where(s1 > 0, lse1, -np.inf)
+    tile_lmin = np.full(T, np.inf)
+    np.minimum.at(tile_lmin, tile, lse1)
+    with np.errstate(divide="ignore"):
+        gap = np.where(s1 > 0, -np.log2(np.where(s1 > 0, s1, 1.0)), -np.inf)
+    tgap = np.full(T, -np.inf)
+    np.maximum.at(tgap, tile, gap)
+    tm0 = np.full(T, -np.inf)
+    np.maximum.at(tm0, tile, np.where(s1 > 0, -np.inf, mx))
+    seed = np.where(tgap >= floor_thr, np.inf, tm0)
+    counts2, lst2 = ref.tiles(scene, cam, ts, tile_lmin=tile_lmin.astype(np.float32),
+                              tile_seed=seed.astype(np.float32))
+    m2, _ = _member(counts2, lst2, T, scene.n)
+    union = m1 | m2
+    complete = union.all(1)
+    mtrue = l2.max(1)
+    _, _, lse_u = lse_of(union[tile])
+    excess = mtrue - lse_u - floor_thr                   # >= 0: the floor is visible there
+    bad = (~complete[tile]) & ~(excess < 0.25)
+    print(f"{kind} ts={ts}: {int(complete.sum())}/{T} tiles completed densely, "
+          f"{int(bad.sum())} pixels with a visible floor on an incomplete tile")
+    assert not np.any(bad)
+
+
+@pytest.mark.parametrize("kind", ["cube_inside", "shell_centre", "wide_fov"])
+@pytest.mark.parametrize("ts", [8, 16, 32])
+def test_masked_logit_bound_holds(scenes_mod, kind, ts):
+    """The membership's bound on the depth logit of a pair where it is masked (t* <= 0), which
+    decides when a tile is completed densely for the 1e-20 floor, is at least the exact float64
+    logit at every pixel of the tile where the pair is masked."""
+    sc = scenes_mod
+    scene, cam = _floor_scene(sc, kind)
+    bad, checked = ref.masked_bound_check(scene, cam, ts)
+    print(f"{kind} ts={ts}: {checked} masked (pixel, Gaussian) logits checked, {bad} violating pairs")
+    assert checked > 0 and bad == 0
