@@ -47,7 +47,7 @@ namespace fe {
 //   g_u = -(g_t/vsv + g_d2 t) v - (g_t/vsv) t u.
 // SAFE zeroes non-finite per-pair terms (views with degenerate records only).
 template <bool SAFE>
-__device__ __forceinline__ void bwd_pixels(const float4* pxa, const float4* pxb, const float* pxc, int npx, float a0,
+__device__ __forceinline__ void bwd_pixels(const float4* pxa, const float4* pxb, const float2* pxc, int npx, float a0,
                                            float a1,
                                            float a2, float bb0, float bb1, float bb2, float g0, float g1, float g2,
                                            float q0, float q1, float q2, float wl, float b3, float b2, float nb2l,
@@ -76,7 +76,7 @@ __device__ __forceinline__ void bwd_pixels(const float4* pxa, const float4* pxb,
     s = (t > 0.0f) ? s : 0.0f;
     const float cs = cD * s;
     const float dt = t - D;
-    const float ex = (l == M) ? pxc[p] : 0.0f;   // the pair holding the maximum logit
+    const float ex = (l == M) ? pxc[p].y : 0.0f;   // the pair holding the maximum logit
     float gd2 = fmaf(b3, ex, fmaf(b3 * cs, dt, cE * e));
     float gtv = fmaf(-b2, ex, fmaf(cs, fmaf(-b2, dt, 1.0f), cF)) * iv;
     if (SAFE && (!(gd2 == gd2) || !(gtv == gtv))) { gd2 = 0.0f; gtv = 0.0f; }
@@ -198,6 +198,7 @@ constexpr int kGsumPlanes = 11;
 // Per-pixel terms of the backward (upstream gradients through the saved aggregates).
 struct PixTerms {
   float xf, yf, M, D, cE, cD, cDb, cF, cX;
+  float A1, A2, B1;   // g_t = s (A1 - A2 t) + cF, g_d2 = s (cDb t + B1) + cE e
   bool floor_px;
 };
 
@@ -222,16 +223,23 @@ __device__ __forceinline__ PixTerms pix_terms(const BwdArgs& A, const Job& jb, c
   float cX = -cD * fmaf(-D, S, Z);
   if (!(cX == cX) || fabsf(cX) > 3.0e38f) cX = 0.0f;
   t.M = M; t.D = D; t.cE = cE; t.cD = cD; t.cDb = jb.beta3 * cD; t.cF = 1e-20f * cD; t.cX = cX;
+  // cD s (1 - b2 (t - D)) = s (A1 - A2 t) and b3 cD s (t - D) = s (cDb t + B1)
+  t.A2 = jb.beta2 * cD;
+  t.A1 = fmaf(t.A2, D, cD);
+  t.B1 = -(t.cDb * D);
+  if (cD == 0.0f || !(t.A1 == t.A1) || !(t.B1 == t.B1)) { t.A1 = 0.0f; t.A2 = 0.0f; t.B1 = 0.0f; }
   t.floor_px = !(S >= 1e-11f * ncnt);
   return t;
 }
 
 // Blend-record loop over pixels [p0, p1) for one Gaussian (this lane); see the header
-// comment.  pxb holds (cE, cD, b3 cD, 1e-20 cD), pxc the maximum-logit term cX.
+// comment.  Used for ray-list tiles: pxa holds (x, y, M, D) with (x, y) the ray's
+// camera-plane coordinates, pxb (cE, cD, b3 cD, 1e-20 cD), pxc (1e-20 cD, the maximum-logit
+// term cX).
 // Sums: G[0..2] = sum g_X p, G[3..5] = sum c Y1 p, G[6..8] = sum c Y2 p, G[9] = sum g_d2,
 // G[10] = sum g_d2 W r.
 template <bool MASK, bool ARGMAX>
-__device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4* pxb, const float* pxc, int p0,
+__device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4* pxb, const float2* pxc, int p0,
                                                 int p1, const float* k, float wL, float cG, float qq, float b3,
                                                 float b2, float kap, float* G) {
   const float kx0 = k[0], kx1 = k[1], kx2 = k[2], k10 = k[3], k11 = k[4], k12 = k[5], k20 = k[6], k21 = k[7],
@@ -256,7 +264,7 @@ __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4*
     float g1 = fmaf(cD * s, fmaf(-b2, dt, 1.0f), cF);     // dL/dt
     float gd2 = fmaf(cDb * s, dt, cE * e);                // dL/dd2
     if (ARGMAX) {
-      const float ex = (b3 * lt == M) ? pxc[p] : 0.0f;   // the pair holding the maximum logit
+      const float ex = (b3 * lt == M) ? pxc[p].y : 0.0f;   // the pair holding the maximum logit
       g1 = fmaf(-b2, ex, g1);
       gd2 = fmaf(b3, ex, gd2);
     }
@@ -271,6 +279,100 @@ __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4*
     G[9] += gd2;
     G[10] = fmaf(gd2, Wr, G[10]);
   }
+}
+
+// Row form of the blend-record loop for pixel-grid tiles (row width wv), pixels [p0, p1).
+// pxa holds (x', x'^2, M, cE) and pxb (A1, A2, b3 cD, B1) with tile-local coordinates
+// x' = ix - hc, y' = iy - hc (hc = (TS-1)/2); pxc (1e-20 cD, cX).  The forms X, Y1, Y2 take
+// their y-part once per row.  The sums go per row and per tile-local moment:
+//   sum g_X p' from sum g_X x' and per-row sums of g_X,
+//   sum c Y_i p' = C k_i' with C = sum c p' p'^T (6 moments) and k_i' the form's local
+//   coefficients, so the per-pixel work does not depend on the two forms Y1, Y2,
+// and are returned for the global p = p' + (ox, oy, 0) in the layout of bwd_pixels_rec2.
+// The argmax variant (tiles where the 1e-20 floor is visible) evaluates X, Y1, Y2 at the
+// global coordinates with the forward's exact operations, so that the pair holding the
+// maximum logit is found by equality; elsewhere they use the tile-local form.
+template <bool MASK, bool ARGMAX>
+__device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* pxb, const float2* pxc, int p0,
+                                              int p1, int wv, float hc, int px0, int py0, float cx, float cy,
+                                              const float* k, float wL, float cG, float qq, float b3, float b2,
+                                              float kap, float* G) {
+  const float kx0 = k[0], kx1 = k[1], k10 = k[3], k11 = k[4], k20 = k[6], k21 = k[7];
+  // global coordinates of the local origin, and the forms' constants there
+  const float ox = ((float)px0 + hc) - cx, oy = ((float)py0 + hc) - cy;
+  const float kxl = fmaf(kx0, ox, fmaf(kx1, oy, k[2]));
+  const float k1l = fmaf(k10, ox, fmaf(k11, oy, k[5]));
+  const float k2l = fmaf(k20, ox, fmaf(k21, oy, k[8]));
+  const float kxr = ARGMAX ? k[2] : kxl, k1r = ARGMAX ? k[5] : k1l, k2r = ARGMAX ? k[8] : k2l;
+  float SgXx = 0.f, SgXy = 0.f, SgX = 0.f;
+  float Cxx = 0.f, Cxy = 0.f, Cyy = 0.f, Cx = 0.f, Cy = 0.f, C1 = 0.f;
+  float S9 = 0.f, S10 = 0.f;
+  int p = p0;
+  while (p < p1) {
+    const int iy = p / wv;
+    const int pe = min(p1, (iy + 1) * wv);
+    const float yl = (float)iy - hc;
+    const float yr = ARGMAX ? (float)(py0 + iy) - cy : yl;
+    const float XR = fmaf(kx1, yr, kxr), Y1R = fmaf(k11, yr, k1r), Y2R = fmaf(k21, yr, k2r);
+    float Rg = 0.f, Rcx = 0.f, Rc = 0.f;
+    for (; p < pe; ++p) {
+      const float4 pa = pxa[p];
+      const float4 pb = pxb[p];
+      const float xl = pa.x, xl2 = pa.y, M = pa.z, cE = pa.w;
+      const float A1 = pb.x, A2 = pb.y, cDb = pb.z, B1 = pb.w;
+      const float xr = ARGMAX ? (float)(px0 + (p - iy * wv)) - cx : xl;
+      const float X = fmaf(kx0, xr, XR);
+      const float Y1 = fmaf(k10, xr, Y1R);
+      const float Y2 = fmaf(k20, xr, Y2R);
+      const float W = fmaf(Y1, Y1, Y2 * Y2);
+      const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
+      const float t = X * rr;
+      const float Wr = W * rr;
+      const float d2l = fmaf(-cG, Wr, wL);
+      const float e = __builtin_amdgcn_exp2f(d2l);
+      const float lt = fmaf(kap, t, d2l);
+      float s = __builtin_amdgcn_exp2f(fmaf(b3, lt, -M));
+      if (MASK) s = (t > 0.0f) ? s : 0.0f;
+      float g1, gd2;
+      if (ARGMAX) {
+        const float2 pc = pxc[p];
+        g1 = fmaf(s, fmaf(-A2, t, A1), pc.x);                  // dL/dt
+        gd2 = fmaf(s, fmaf(cDb, t, B1), cE * e);                // dL/dd2
+        const float ex = (b3 * lt == M) ? pc.y : 0.0f;          // the pair holding the maximum logit
+        g1 = fmaf(-b2, ex, g1);
+        gd2 = fmaf(b3, ex, gd2);
+      } else {
+        // the floor term 1e-20 cD of dL/dt is below 1e-9 of the rest on these tiles
+        g1 = s * fmaf(-A2, t, A1);
+        gd2 = fmaf(s, fmaf(cDb, t, B1), cE * e);
+      }
+      const float Gq = gd2 * qq;
+      const float h = fmaf(Gq, X, g1 + g1);
+      const float gX = fmaf(-(t * t), h, fmaf(Gq, t, g1 * rr));
+      const float c = h * (t * rr);
+      SgXx = fmaf(gX, xl, SgXx); Rg += gX;
+      Cxx = fmaf(c, xl2, Cxx); Rcx = fmaf(c, xl, Rcx); Rc += c;
+      S9 += gd2;
+      S10 = fmaf(gd2, Wr, S10);
+    }
+    SgX += Rg; SgXy = fmaf(yl, Rg, SgXy);
+    Cx += Rcx; Cxy = fmaf(yl, Rcx, Cxy);
+    C1 += Rc; Cy = fmaf(yl, Rc, Cy); Cyy = fmaf(yl * yl, Rc, Cyy);
+  }
+  G[0] = fmaf(ox, SgX, SgXx); G[1] = fmaf(oy, SgX, SgXy); G[2] = SgX;
+  {
+    const float sx = fmaf(k10, Cxx, fmaf(k11, Cxy, k1l * Cx));
+    const float sy = fmaf(k10, Cxy, fmaf(k11, Cyy, k1l * Cy));
+    const float s1 = fmaf(k10, Cx, fmaf(k11, Cy, k1l * C1));
+    G[3] = fmaf(ox, s1, sx); G[4] = fmaf(oy, s1, sy); G[5] = s1;
+  }
+  {
+    const float sx = fmaf(k20, Cxx, fmaf(k21, Cxy, k2l * Cx));
+    const float sy = fmaf(k20, Cxy, fmaf(k21, Cyy, k2l * Cy));
+    const float s1 = fmaf(k20, Cx, fmaf(k21, Cy, k2l * C1));
+    G[6] = fmaf(ox, s1, sx); G[7] = fmaf(oy, s1, sy); G[8] = s1;
+  }
+  G[9] = S9; G[10] = S10;
 }
 
 // One block (4 waves) per tile; lane = one Gaussian of the tile's lists (phase 1 then
@@ -290,7 +392,7 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   constexpr int NPIX = TS * TS;
   __shared__ float4 pxa[NPIX];   // x', y', M, D
   __shared__ float4 pxb[NPIX];   // cE, cD, b3 cD, 1e-20 cD
-  __shared__ float pxc[NPIX];    // cX (maximum-logit term)
+  __shared__ float2 pxc[NPIX];   // 1e-20 cD, cX (maximum-logit term)
   __shared__ float red[4][11][64];
 
   const int64_t gt = blockIdx.x;
@@ -312,24 +414,35 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   if (total == 0) return;
   const float ncnt = (float)total;
 
+  const int32_t jflags = A.job_flags[ji];
+  const bool rec2 = use_rec2(jb, jflags);
+  const bool rows = rec2 && !jb.ray_xy;   // pixel-grid tile: row form, tile-local coordinates
+  const float hc = 0.5f * (float)(TS - 1);
   bool floor_px = false;
   for (int i = threadIdx.x; i < npx; i += 256) {
-    const int px = tx * TS + (i % wv), py = ty * TS + (i / wv);
+    const int ix = i % wv, iy = i / wv;
+    const int px = tx * TS + ix, py = ty * TS + iy;
     const int64_t lp = (int64_t)py * jb.W + px;
     const PixTerms P = pix_terms(A, jb, bj, lp, px, py, ncnt);
     floor_px = floor_px || P.floor_px;
-    pxa[i] = make_float4(P.xf, P.yf, P.M, P.D);
-    pxb[i] = make_float4(P.cE, P.cD, P.cDb, P.cF);
-    pxc[i] = P.cX;
+    if (rows) {
+      const float xl = (float)ix - hc;
+      pxa[i] = make_float4(xl, xl * xl, P.M, P.cE);
+      pxb[i] = make_float4(P.A1, P.A2, P.cDb, P.B1);
+    } else {
+      pxa[i] = make_float4(P.xf, P.yf, P.M, P.D);
+      pxb[i] = make_float4(P.cE, P.cD, P.cDb, P.cF);
+    }
+    pxc[i] = make_float2(P.cF, P.cX);
   }
   const bool floor_tile = __syncthreads_or(floor_px ? 1 : 0) != 0;
 
   const float b3 = jb.beta3, b2 = jb.beta2;
   const float nb2l = -jb.beta2 * kLog2e;
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int32_t jflags = A.job_flags[ji];
+  // wave index as a wave-uniform value: each wave's pixel range then runs as a scalar loop
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
 
-  if (use_rec2(jb, jflags)) {
+  if (rec2) {
     const bool masked = A.dense || !A.tile_np || (A.tile_np[2 * gt] | A.tile_np[2 * gt + 1]) != 0;
     const float kap = nb2l / b3;
     const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
@@ -362,9 +475,23 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
       for (int k = 0; k < 11; ++k) G[k] = 0.0f;
       if (act) {
         const float k9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
-        if (masked && floor_tile) bwd_pixels_rec2<true, true>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
-        else if (masked) bwd_pixels_rec2<true, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
-        else bwd_pixels_rec2<false, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
+        if (rows) {
+          const int px0 = tx * TS, py0 = ty * TS;
+          if (masked && floor_tile)
+            bwd_rows_rec2<true, true>(pxa, pxb, pxc, p0, p1, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y, q2.z, q2.w,
+                                      b3, b2, kap, G);
+          else if (masked)
+            bwd_rows_rec2<true, false>(pxa, pxb, pxc, p0, p1, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y, q2.z, q2.w,
+                                       b3, b2, kap, G);
+          else
+            bwd_rows_rec2<false, false>(pxa, pxb, pxc, p0, p1, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y, q2.z,
+                                        q2.w, b3, b2, kap, G);
+        } else {
+          if (masked && floor_tile)
+            bwd_pixels_rec2<true, true>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
+          else if (masked) bwd_pixels_rec2<true, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
+          else bwd_pixels_rec2<false, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
+        }
       }
 #pragma unroll
       for (int k = 0; k < 11; ++k) red[wave][k][lane] = G[k];
