@@ -218,13 +218,14 @@ class Renderer:
 
     def debug_state(self) -> dict:
         """Host copies of the last forward's workspace internals (tests only): per-Gaussian
-        records, tile ranges and lists of both phases, per-tile phase-1 thresholds and floor
-        seeds."""
+        records and their order (record j of a view is Gaussian order[j]), tile ranges and
+        lists of both phases (record ids), per-tile phase-1 thresholds and floor seeds."""
         torch.cuda.synchronize(self.device)
         ws = self._ws.cpu().numpy()
-        h = ws[:8 * 12].view(np.uint64)
-        magic, o_cnt, o_off1, o_off2, T, mode, o_rec, o_list, o_lmin, n_rec, o_st, n_pix = [int(x) for x in h]
-        out = {"records": ws[o_rec:o_rec + 64 * n_rec].view(np.float32).reshape(n_rec, 16), "mode": mode}
+        h = ws[:8 * 13].view(np.uint64)
+        magic, o_cnt, o_off1, o_off2, T, mode, o_rec, o_list, o_lmin, n_rec, o_st, n_pix, o_ord = [int(x) for x in h]
+        out = {"records": ws[o_rec:o_rec + 64 * n_rec].view(np.float32).reshape(n_rec, 16), "mode": mode,
+               "order": ws[o_ord:o_ord + 4 * n_rec].view(np.int32).copy()}
         if mode == _lib.FE_MODE_CULLED:
             off1 = ws[o_off1:o_off1 + 8 * (T + 1)].view(np.uint64).astype(np.int64)
             off2 = ws[o_off2:o_off2 + 8 * (T + 1)].view(np.uint64).astype(np.int64)

@@ -95,9 +95,9 @@ __device__ __forceinline__ void bwd_pixels(const float4* pxa, const float4* pxb,
   }
 }
 
-// Chain rule of the blend-record sums for Gaussian gid of view jb into the scene gradients
-// (10 atomics).  Float32 (float64 here doubles the kernel's registers and halves its
-// occupancy); against a float64 chain it differs by ~1e-6 of the largest gradient.
+// Chain rule of the blend-record sums of Gaussian gid (scene index) of view jb into its scene
+// gradients (10 atomics).  Float32 (float64 here doubles the kernel's registers and halves
+// its occupancy); against a float64 chain it differs by ~1e-6 of the largest gradient.
 __device__ __forceinline__ void rec2_chain(const Job& jb, const BwdJob& bj, int64_t gid, const float* GX,
                                            const float* GY1, const float* GY2, float Gw, float Gqq) {
   const float* P = jb.prec + 9 * gid;
@@ -545,8 +545,9 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     // chain rule for this (tile, Gaussian) pair, straight into the scene gradients:
     //   dM = [G_a/fx, G_b/fy, G_g],  dA = dM R + G_q p^T (lower part),  dmu = A^T G_q,
     //   d(prec)[k][i] += dA[i][k] for k <= i,  dw += G_w
-    const float* Pp = jb.prec + 9 * gid;
-    const float* mu = jb.means + 3 * gid;
+    const int64_t sg = gauss_of(jb, gid);   // the record's Gaussian in the scene arrays
+    const float* Pp = jb.prec + 9 * sg;
+    const float* mu = jb.means + 3 * sg;
     const float u00 = Pp[0], u01 = Pp[1], u02 = Pp[2], u11 = Pp[4], u12 = Pp[5], u22 = Pp[8];
     const float pv[3] = {mu[0] - jb.o[0], mu[1] - jb.o[1], mu[2] - jb.o[2]};
     const float dm[3][3] = {{Ga0 * ifx, Gb0 * ify, Gg0}, {Ga1 * ifx, Gb1 * ify, Gg1}, {Ga2 * ifx, Gb2 * ify, Gg2}};
@@ -558,13 +559,13 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
       for (int k = 0; k <= i; ++k)
         dA[i][k] = fmaf(dm[i][0], jb.R[k], fmaf(dm[i][1], jb.R[3 + k], fmaf(dm[i][2], jb.R[6 + k], gq[i] * pv[k])));
     if (bj.dmeans) {
-      float* dmu = bj.dmeans + 3 * gid;
+      float* dmu = bj.dmeans + 3 * sg;
       atomicAdd(dmu + 0, fmaf(u00, Gq0, fmaf(u01, Gq1, u02 * Gq2)));
       atomicAdd(dmu + 1, fmaf(u11, Gq1, u12 * Gq2));
       atomicAdd(dmu + 2, u22 * Gq2);
     }
     if (bj.dprec) {
-      float* dp = bj.dprec + 9 * gid;
+      float* dp = bj.dprec + 9 * sg;
       atomicAdd(dp + 0, dA[0][0]);
       atomicAdd(dp + 1, dA[1][0]);
       atomicAdd(dp + 2, dA[2][0]);
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
       atomicAdd(dp + 5, dA[2][1]);
       atomicAdd(dp + 8, dA[2][2]);
     }
-    if (bj.dw) atomicAdd(bj.dw + gid, Gw);
+    if (bj.dw) atomicAdd(bj.dw + sg, Gw);
   }
 }
 
@@ -586,12 +587,15 @@ __global__ void k_fill_zero(float4* p, int64_t n4) {
 __global__ __launch_bounds__(256) void k_bwd_chain(const Job* __restrict__ jobs, const BwdJob* __restrict__ bjobs,
                                                    int n_jobs, const int32_t* __restrict__ job_flags,
                                                    const float* __restrict__ gsum, int64_t total_rec) {
-  const int64_t r = blockIdx.x * 256ll + threadIdx.x;
-  if (r >= total_rec) return;
-  const int ji = find_job(n_jobs, r, [&](int k) { return jobs[k].rec_base; });
+  // one thread per (view, Gaussian) in scene order, so the gradient atomics of a wave fall on
+  // consecutive Gaussians; the sums are read from the Gaussian's record
+  const int64_t i = blockIdx.x * 256ll + threadIdx.x;
+  if (i >= total_rec) return;
+  const int ji = find_job(n_jobs, i, [&](int k) { return jobs[k].rec_base; });
   const Job& jb = jobs[ji];
-  const int64_t j = r - jb.rec_base;
-  if (j >= jb.n || !use_rec2(jb, job_flags[ji])) return;
+  const int64_t gid = i - jb.rec_base;
+  if (gid >= jb.n || !use_rec2(jb, job_flags[ji])) return;
+  const int64_t r = jb.rec_base + (jb.rank ? jb.rank[gid] : gid);
   float g[kGsumPlanes];
   bool any = false;
 #pragma unroll
@@ -600,7 +604,7 @@ __global__ __launch_bounds__(256) void k_bwd_chain(const Job* __restrict__ jobs,
     any = any || g[k] != 0.0f;
   }
   if (!any) return;   // no tile listed this Gaussian
-  rec2_chain(jb, bjobs[ji], j, g, g + 3, g + 6, g[9], g[10]);
+  rec2_chain(jb, bjobs[ji], gid, g, g + 3, g + 6, g[9], g[10]);
 }
 
 }  // namespace fe

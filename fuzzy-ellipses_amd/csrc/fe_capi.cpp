@@ -49,7 +49,7 @@ struct Layout {
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_rec2 = 0, o_tnp = 0,
          o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
-         o_bjobs = 0, o_gsum = 0, o_rect = 0, bytes = 0;
+         o_bjobs = 0, o_gsum = 0, o_rect = 0, o_order = 0, o_rank = 0, bytes = 0;
 };
 
 // A small header at the start of the workspace lets fe_read_stats find the counters.
@@ -57,6 +57,7 @@ struct WsHeader {
   uint64_t magic;
   uint64_t o_counters, o_off1, o_off2, total_tiles, mode;
   uint64_t o_rec, o_list, o_lmin, total_rec, o_state, total_pix;
+  uint64_t o_order;
 };
 constexpr uint64_t kMagic = 0x4645414d44313031ull;  // "FEAMD101"
 
@@ -195,6 +196,8 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->o_flags = p; p = align256(p + sizeof(int32_t) * n_views);
   L->o_rec = p; p = align256(p + sizeof(float) * kRecFloats * (size_t)std::max<int64_t>(rec, 1));
   L->o_rec2 = p; p = align256(p + sizeof(float) * kRec2Floats * (size_t)std::max<int64_t>(rec, 1));
+  L->o_order = p; p = align256(p + sizeof(int32_t) * (size_t)std::max<int64_t>(rec, 1));
+  L->o_rank = p; p = align256(p + sizeof(int32_t) * (size_t)std::max<int64_t>(rec, 1));
   L->o_lmin = p; p = align256(p + sizeof(float) * 2 * tiles);   // thresholds, then floor seeds
   L->o_ubm = p; p = align256(p + sizeof(float) * tiles);
   L->o_state = p; p = align256(p + sizeof(float) * 6 * pix);
@@ -326,12 +329,16 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
       return FE_ERR_INVALID_ARGUMENT;
     }
   }
-  for (int v = 0; v < n_views; ++v)
+  for (int v = 0; v < n_views; ++v) {
     if (jobs[v].ray_xy) jobs[v].rect = at<float>(ws, L.o_rect) + 4 * jobs[v].tile_base;
+    jobs[v].order = at<int32_t>(ws, L.o_order) + jobs[v].rec_base;
+    jobs[v].rank = at<int32_t>(ws, L.o_rank) + jobs[v].rec_base;
+  }
   // header + jobs in one upload
   std::vector<char> blob(L.o_counters);
   WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode,
-             L.o_rec, L.o_list, L.o_lmin, (uint64_t)L.total_rec, L.o_state, (uint64_t)L.total_pix};
+             L.o_rec, L.o_list, L.o_lmin, (uint64_t)L.total_rec, L.o_state, (uint64_t)L.total_pix,
+             L.o_order};
   memcpy(blob.data() + L.o_header, &h, sizeof(h));
   memcpy(blob.data() + L.o_jobs, jobs.data(), sizeof(Job) * n_views);
   if (!L.reuse) {
@@ -352,7 +359,10 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   bool any_pose = false;
   for (int v = 0; v < n_views; ++v) any_pose = any_pose || views[v].pose_device != nullptr;
   if (any_pose) FE_HIP(launch_pose(s, at<Job>(ws, L.o_jobs), L.n_jobs));
-  FE_HIP(timed(s, FE_K_PREPROCESS, [&] { return launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, rec2, flags); }));
+  FE_HIP(timed(s, FE_K_PREPROCESS, [&] {
+    const hipError_t e = launch_order(s, dj, L.n_jobs, L.max_n);
+    return e != hipSuccess ? e : launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, rec2, flags);
+  }));
   if (L.has_rays) FE_HIP(launch_ray_rects(s, dj, L.n_jobs, L.total_tiles, L.ts, flags));
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {

@@ -95,7 +95,15 @@ struct Job {
   const float* ray_xy;
   float* rect;
   int64_t npix;       // valid pixels: H*W, or the number of rays
+  // Record order: record j of this view (and every list id) is Gaussian order[j].  Written
+  // by k_order each forward: the Gaussians sorted by a Z-order code of their projected
+  // centres, in chunks of 1024, so a tile's list is a few runs of nearby ids.
+  int32_t* order;
+  int32_t* rank;      // inverse of order: Gaussian g is record rank[g]
 };
+
+// Scene index of record j of a view.
+__device__ __forceinline__ int64_t gauss_of(const Job& jb, int64_t j) { return jb.order ? jb.order[j] : j; }
 
 // Per-view backward buffers: upstream gradients of the images and the scene's gradients.
 struct BwdJob {

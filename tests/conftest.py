@@ -23,6 +23,16 @@ def load_package():
     return mod
 
 
+def in_record_order(scene, order):
+    """The scene with its Gaussians in the renderer's record order of one view (record j is
+    Gaussian order[j], from Renderer.debug_state); the order must be a permutation."""
+    import dataclasses
+    import numpy as np
+    assert np.array_equal(np.sort(order), np.arange(scene.n)), "record order is not a permutation"
+    return dataclasses.replace(scene, means=scene.means[order], prec=scene.prec[order],
+                               weights_log=scene.weights_log[order])
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through the HIP C ABI)")
 

@@ -69,6 +69,8 @@ def test_ray_list_keys_bit_exact(fea, scenes_mod):
     r.forward([fea.SceneTensors.from_scene(scene)], [(0, rl, scene.beta2, scene.beta3)])
     torch.cuda.synchronize()
     st = r.debug_state()
+    from conftest import in_record_order
+    scene = in_record_order(scene, st["order"])
     off1, off2, lst = st["off1"], st["off2"], st["list"]
     c1, l1 = ref.tiles_rays(scene, cam.R, cam.t, xy, 16)
     np.testing.assert_array_equal(np.diff(off1), c1.astype(np.int64))

@@ -3,6 +3,7 @@ Gaussian lists of both culling phases equal the CPU restatement (oracle/fe_ref.c
 import numpy as np
 import pytest
 
+from conftest import in_record_order
 from oracle import ref
 
 pytestmark = pytest.mark.gpu
@@ -14,6 +15,8 @@ def _check(fea, scene, cam, ts):
     r.forward([fea.SceneTensors.from_scene(scene)], [(0, cam, scene.beta2, scene.beta3)])
     torch.cuda.synchronize()
     st = r.debug_state()
+    # the GPU lists record ids; the restatement runs on the Gaussians in that order
+    scene = in_record_order(scene, st["order"])
     rec_cpu, _ = ref.records(scene, cam)
     np.testing.assert_array_equal(st["records"].view(np.uint32), rec_cpu.view(np.uint32))
     off1, off2, lst = st["off1"], st["off2"], st["list"]
