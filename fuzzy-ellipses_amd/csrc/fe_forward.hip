@@ -802,7 +802,7 @@ __device__ __forceinline__ void blend_batch(const float4* lds, int nbr, float xf
 // the running maximum Mt is of lt and the reference M = b3*Mt is what S and Z are relative
 // to (the value saved for the second pass and the backward).  MASK applies the t* > 0
 // test; a tile whose listed pairs are all in front of the camera skips it.
-template <int SUB, bool MASK, int PPT>
+template <int SUB, bool MASK, int PPT, bool ROW>
 __device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, const float* xf, const float* yf, float b3,
                                                  float kap, float* M, float* Mt, float* S, float* Z, float* E,
                                                  float* Zall) {
@@ -817,11 +817,16 @@ __device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, con
       const float4 r1 = lds[(k0 + s) * 3 + 1];
       const float4 r2 = lds[(k0 + s) * 3 + 2];
       // kX = r0.xyz, kY1 = (r0.w, r1.x, r1.y), kY2 = (r1.z, r1.w, r2.x), wL = r2.y, cG = r2.z
+      // ROW: the thread's pixels share y, so the forms' y-parts are computed once (the same
+      // operations as per pixel, so the values are identical)
+      const float RX = ROW ? fmaf(r0.y, yf[0], r0.z) : 0.0f;
+      const float RY1 = ROW ? fmaf(r1.x, yf[0], r1.y) : 0.0f;
+      const float RY2 = ROW ? fmaf(r1.w, yf[0], r2.x) : 0.0f;
 #pragma unroll
       for (int q = 0; q < PPT; ++q) {
-        const float X = fmaf(r0.x, xf[q], fmaf(r0.y, yf[q], r0.z));
-        const float Y1 = fmaf(r0.w, xf[q], fmaf(r1.x, yf[q], r1.y));
-        const float Y2 = fmaf(r1.z, xf[q], fmaf(r1.w, yf[q], r2.x));
+        const float X = fmaf(r0.x, xf[q], ROW ? RX : fmaf(r0.y, yf[q], r0.z));
+        const float Y1 = fmaf(r0.w, xf[q], ROW ? RY1 : fmaf(r1.x, yf[q], r1.y));
+        const float Y2 = fmaf(r1.z, xf[q], ROW ? RY2 : fmaf(r1.w, yf[q], r2.x));
         const float W = fmaf(Y1, Y1, Y2 * Y2);
         const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
         const float t = X * rr;
@@ -901,13 +906,14 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
   const Job& jb = A.jobs[ji];
   const int tile = (int)(gt - jb.tile_base);
   const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
-  // pixel q of this thread: tile pixel index threadIdx.x + q*NT
+  // pixel q of this thread: tile pixel index PPT*threadIdx.x + q, so a thread's pixels are
+  // neighbours in one row (same y: the blend records' y-parts are shared between them)
   bool inimg[PPT];
   int64_t pix[PPT];
   float xf[PPT], yf[PPT];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
-    const int i = threadIdx.x + q * NT;
+    const int i = PPT * threadIdx.x + q;
     const int px = tx * TS + (i % TS);
     const int py = ty * TS + (i / TS);
     const int64_t lp = (int64_t)py * jb.W + px;
@@ -975,8 +981,13 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
         lds[i] = v;
       }
       __syncthreads();
-      if (masked) blend_batch_rec2<SUB, true, PPT>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-      else blend_batch_rec2<SUB, false, PPT>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+      if (PPT > 1 && !jb.ray_xy) {
+        if (masked) blend_batch_rec2<SUB, true, PPT, true>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        else blend_batch_rec2<SUB, false, PPT, true>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+      } else {
+        if (masked) blend_batch_rec2<SUB, true, PPT, false>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        else blend_batch_rec2<SUB, false, PPT, false>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+      }
       __syncthreads();
       continue;
     }
