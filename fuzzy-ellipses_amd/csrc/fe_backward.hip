@@ -395,7 +395,7 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   __shared__ float2 pxc[NPIX];   // 1e-20 cD, cX (maximum-logit term)
   __shared__ float red[4][11][64];
 
-  const int64_t gt = blockIdx.x;
+  const int64_t gt = xcd_tile(blockIdx.x, gridDim.x);   // one block per tile
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
   const Job& jb = A.jobs[ji];
   const BwdJob& bj = A.bjobs[ji];

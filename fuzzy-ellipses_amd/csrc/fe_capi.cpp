@@ -513,9 +513,11 @@ fe_status fe_read_stats(void* stream, const void* workspace, size_t workspace_by
   FE_HIP(hipStreamSynchronize(s));
   out->pairs_phase1 = (int64_t)t1;
   out->pairs_phase2 = (int64_t)(t2 - t1);
-  out->evals = (int64_t)c.evals;
+  unsigned long long ev = 0, ev1 = 0;
+  for (int k = 0; k < fe::kCountSlots; ++k) { ev += c.slot[k][0]; ev1 += c.slot[k][1]; }
+  out->evals = (int64_t)ev;
   out->floor_tiles = (int64_t)c.floor_tiles;
-  out->evals_phase1 = (int64_t)c.evals1;
+  out->evals_phase1 = (int64_t)ev1;
   out->floor_unresolved = (int64_t)c.floor_unresolved;
   return FE_OK;
 }

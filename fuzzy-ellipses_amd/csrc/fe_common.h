@@ -102,6 +102,17 @@ struct Job {
   int32_t* rank;      // inverse of order: Gaussian g is record rank[g]
 };
 
+// Tile of a block of a one-block-per-tile launch of n blocks.  The hardware deals blocks
+// round-robin to the 8 XCDs (block b to XCD b % 8), each with its own L2; this gives XCD x
+// one contiguous run of tiles instead of every 8th, so neighbouring tiles (which share the
+// cache lines of their pixel rows and most of their records) run on the same L2.  A
+// bijection of [0, n) whatever the dispatch does.
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
+  const int64_t per = n >> 3, rem = n & 7;
+  const int64_t x = b & 7, k = b >> 3;
+  return x * per + (x < rem ? x : rem) + k;
+}
+
 // Scene index of record j of a view.
 __device__ __forceinline__ int64_t gauss_of(const Job& jb, int64_t j) { return jb.order ? jb.order[j] : j; }
 
@@ -114,13 +125,16 @@ struct BwdJob {
   float* dw;
 };
 
-// Device counters (zeroed by k_init each forward).
+// Device counters (zeroed by k_init each forward).  The per-block evaluation counts go to
+// one of kCountSlots slots, each on its own 128-B line, picked by block index: a single
+// counter would take one same-address atomic per blend block (19k per 16-view launch), and
+// those serialise at memory (they bounded the phase-2 blend).  Readers add the slots.
+constexpr int kCountSlots = 64;
 struct Counters {
-  unsigned long long evals;
   unsigned long long floor_tiles;       // tiles completed densely because the floor could matter
-  unsigned long long evals1;
   unsigned long long floor_unresolved;  // pixels' tiles where it still could (must stay 0)
-  unsigned long long pad[4];
+  unsigned long long pad[14];
+  unsigned long long slot[kCountSlots][16];   // [i][0]: evaluated pairs, [i][1]: of them phase 1
 };
 
 // Whether a view's blends use the blend records: no degenerate record, every record valid,

@@ -51,8 +51,9 @@ __device__ __forceinline__ float block_ext(float v, float* sh) {
 // k_init
 // ------------------------------------------------------------------------------------------
 __global__ void k_init(Counters* cnt, int32_t* job_flags, int n_jobs) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < (int)(sizeof(Counters) / 8)) reinterpret_cast<unsigned long long*>(cnt)[i] = 0ull;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  for (int k = i; k < (int)(sizeof(Counters) / 8); k += gridDim.x * blockDim.x)
+    reinterpret_cast<unsigned long long*>(cnt)[k] = 0ull;
   for (int k = i; k < n_jobs; k += gridDim.x * blockDim.x) job_flags[k] = 0;
 }
 
@@ -901,7 +902,7 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
   __shared__ float4 lds[GB * 4];
   __shared__ float sh_red[NT / 64 + 1];
 
-  const int64_t gt = blockIdx.x;
+  const int64_t gt = xcd_tile(blockIdx.x, gridDim.x);   // one block per tile
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
   const Job& jb = A.jobs[ji];
   const int tile = (int)(gt - jb.tile_base);
@@ -1047,8 +1048,9 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
     if (threadIdx.x == 0) {
       A.tile_lmin[gt] = lm;
       A.tile_seed[gt] = (gap >= jb.floor_thr) ? INFINITY : m0;
-      atomicAdd(&A.cnt->evals, ev);
-      atomicAdd(&A.cnt->evals1, ev);
+      unsigned long long* c = A.cnt->slot[blockIdx.x % kCountSlots];
+      atomicAdd(c, ev);
+      atomicAdd(c + 1, ev);
     }
     return;
   }
@@ -1092,12 +1094,13 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
     const int any = __syncthreads_or(fl ? 1 : 0);
     if (threadIdx.x == 0) {
       if (any) atomicAdd(&A.cnt->floor_unresolved, 1ull);
-      atomicAdd(&A.cnt->evals, ev);
+      atomicAdd(&A.cnt->slot[blockIdx.x % kCountSlots][0], ev);
     }
   } else {
     if (threadIdx.x == 0) {
-      atomicAdd(&A.cnt->evals, ev);
-      atomicAdd(&A.cnt->evals1, ev);
+      unsigned long long* c = A.cnt->slot[blockIdx.x % kCountSlots];
+      atomicAdd(c, ev);
+      atomicAdd(c + 1, ev);
     }
   }
 }
