@@ -712,24 +712,62 @@ __global__ __launch_bounds__(256) void k_scan_final(const uint32_t* __restrict__
 // ------------------------------------------------------------------------------------------
 // in-order compaction: one wave per tile, 4 tiles per block
 // ------------------------------------------------------------------------------------------
+// Position of the j-th set bit (j < popcount) of a 64-bit word, by halving.
+__device__ __forceinline__ int select_bit(unsigned long long w, int j) {
+  int pos = 0;
+#pragma unroll
+  for (int half = 32; half > 0; half >>= 1) {
+    const unsigned long long lo = w & ((1ull << half) - 1ull);
+    const int c = __popcll(lo);
+    if (j >= c) { j -= c; w >>= half; pos += half; }
+    else w = lo;
+  }
+  return pos;
+}
+
 __global__ __launch_bounds__(256) void k_tile_write(const Job* __restrict__ jobs, int n_jobs,
                                                     int64_t total_tiles,
                                                     const unsigned long long* __restrict__ mask,
                                                     const unsigned long long* __restrict__ off,
                                                     uint32_t* __restrict__ list) {
+  // One wave per tile.  Per window of 64 membership words: lane c loads word c, the wave
+  // scans the words' popcounts, and then lane k of each round writes list entry k of the
+  // window: its word by binary search on the inclusive counts (in LDS), its bit by select.
+  // The stores of a round are consecutive.
+  __shared__ unsigned long long sw[4][64];
+  __shared__ uint32_t sinc[4][64];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int64_t gt = (int64_t)blockIdx.x * 4 + wave;
-  if (gt >= total_tiles) return;
+  if (gt >= total_tiles) return;   // wave-uniform; no block barrier below
   const int ji = find_job(n_jobs, gt, [&](int k) { return jobs[k].tile_base; });
   const Job& jb = jobs[ji];
   const int tile = (int)(gt - jb.tile_base);
   const unsigned long long* mrow = mask + jb.mask_base + (int64_t)tile * jb.nw;
   unsigned long long pos = off[gt];
-  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int c = 0; c < jb.nw; ++c) {
-    const unsigned long long word = mrow[c];
-    if ((word >> lane) & 1ull) list[pos + __popcll(word & lt)] = (uint32_t)(c * 64 + lane);
-    pos += __popcll(word);
+  for (int c0 = 0; c0 < jb.nw; c0 += 64) {
+    const int c = c0 + lane;
+    const unsigned long long word = (c < jb.nw) ? mrow[c] : 0ull;
+    uint32_t inc = (uint32_t)__popcll(word);
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)inc, o, 64);
+      if (lane >= o) inc += v;
+    }
+    sw[wave][lane] = word;
+    sinc[wave][lane] = inc;
+    const uint32_t tot = (uint32_t)__shfl((int)inc, 63, 64);
+    __builtin_amdgcn_wave_barrier();
+    for (uint32_t k = lane; k < tot; k += 64) {
+      // first word whose inclusive count exceeds k
+      int lo = 0;
+#pragma unroll
+      for (int step = 32; step > 0; step >>= 1)
+        if (sinc[wave][lo + step - 1] <= k) lo += step;
+      const uint32_t before = sinc[wave][lo] - (uint32_t)__popcll(sw[wave][lo]);
+      list[pos + k] = (uint32_t)((c0 + lo) * 64 + select_bit(sw[wave][lo], (int)(k - before)));
+    }
+    pos += tot;
+    __builtin_amdgcn_wave_barrier();
   }
 }
 
