@@ -211,16 +211,19 @@ __device__ __forceinline__ float4 ray_rect(float xmin, float xmax, float ymin, f
 
 // Conservative bounds of one Gaussian over one tile's pixel rays.
 //   sd_lo, sd_hi : bounds of sqrt(d0) (distance of q to the ray line in whitened space)
-//   t_lb         : lower bound of t* where the whole tile lies on the q side (else 0)
+//   t_lb         : lower bound of t* where the whole tile lies on the q side, else 0
+//                  (pair_t_lb)
 //   plus         : true when every ray of the tile has q.u > 0 (t* > 0 on the tile)
-//   t_any_lb     : lower bound of t* on the tile regardless of side (-inf if unknown)
-//   t_ub         : upper bound of t* on the tile (+inf if unknown)
+//   t_any_lb     : lower bound of t* on the tile regardless of side, -inf if unknown
+//                  (inside masked_logit_ub)
+//   t_ub         : upper bound of t* on the tile, +inf if unknown (pair_t_ub)
 //   unbounded    : the tile cone is too wide for the bound (keep the pair)
 // sqrt(d0) = |q| sin(angle(q, line)) is |q|-Lipschitz in the ray angle; the angle from the
 // centre ray to any tile ray is at most the largest corner angle, bounded by its tangent.
 //   front        : like plus, with a 2^-10 relative margin on sd_hi < |q| (used only to let
 //                  the blend skip the t* > 0 test on a tile, never for membership)
-//   can_mask, d0m_lo, negt_hi : where a tile ray can have t* <= 0, the angle theta between it
+//   can_mask, d0m_lo, negt_hi (inside masked_logit_ub) : where a tile ray can have t* <= 0,
+//                  the angle theta between it
 //                  and q is at least 90 deg; with theta_c the centre ray's angle and phi the
 //                  tile's angular radius, theta <= theta_c + phi there, so
 //                  d0 = |q|^2 sin^2(theta) >= d0m_lo and -t* <= negt_hi on the masked rays
@@ -228,11 +231,15 @@ __device__ __forceinline__ float4 ray_rect(float xmin, float xmax, float ymin, f
 //                  masked pair far below the one from (sd_lo, t_any_lb), whose extremes
 //                  cannot occur at the same ray.
 struct PairBound {
-  float sd_lo, sd_hi, t_lb, t_any_lb, t_ub;
-  float d0m_lo, negt_hi;
-  bool plus, unbounded, front, can_mask;
+  float sd_lo, sd_hi;           // bounds of sqrt(d0) on the tile
+  float qn, qu, sdc, tanm, dth; // |q|, q.u_c, sqrt(d0) at the centre ray, tan of the half-angle
+  float uu, ap2, qq;            // |u_c|^2, the largest |a hx +- b hy|^2, |q|^2
+  bool plus, unbounded, front;
 };
 
+// Core of the bound of a (tile, Gaussian) pair.  The parts that only some tests need (t_lb,
+// t_ub, the masked-logit bound) are computed on demand from the stored intermediates, with
+// the same operations as an eager evaluation, so every value is the same either way.
 __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& tg) {
   const float* a = r;
   const float* b = r + 3;
@@ -271,38 +278,59 @@ __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& t
   const float dth = qn * tanm;
   pb.sd_lo = fmaxf(sdc - dth, 0.0f);
   pb.sd_hi = sdc + dth;
-  const float rho = sqrtf(fmaxf(dot3(ap, ap), dot3(am, am)));
-  const float un = sqrtf(uu);
+  pb.qn = qn; pb.qu = qu; pb.sdc = sdc; pb.tanm = tanm; pb.dth = dth;
+  pb.uu = uu; pb.ap2 = fmaxf(dot3(ap, ap), dot3(am, am)); pb.qq = qq;
   pb.plus = (qu > 0.0f) && (pb.sd_hi < qn);
   pb.front = (qu > 0.0f) && (pb.sd_hi * 1.0009765625f < qn);
-  pb.t_lb = pb.plus ? sqrtf(fmaxf(fmaf(-pb.sd_hi, pb.sd_hi, qq), 0.0f)) / (un + rho) : 0.0f;
-  // |q.u|/|u| <= sqrt(qq - d0_lo) on the tile, |u| >= |u_c| - rho
-  pb.t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-pb.sd_lo, pb.sd_lo, qq), 0.0f)) / (un - rho)) : -INFINITY;
-  // upper bound of t* on the tile: q.u/|u| <= min(|q|, q.u_c/|u_c| + |q| tan(angle)), |u| >= |u_c| - rho
-  pb.t_ub = (un > rho) ? fminf(qn, qu / un + dth) / (un - rho) : INFINITY;
-  // masked part of the tile cone: cos/sin of theta_c + phi from cos/sin of each
-  const float cth = qu / (qn * un);
-  const float sth = sdc / qn;
-  const float cph = 1.0f / sqrtf(fmaf(tanm, tanm, 1.0f));
-  const float sph = tanm * cph;
-  const float cmax = fmaf(cth, cph, -(sth * sph));
-  const float smax = fmaf(sth, cph, cth * sph);
-  pb.can_mask = !(cmax > 0.0f);
-  // past 180 deg (smax < 0) the cone holds the ray straight away from q: sin^2 >= 0, -cos <= 1
-  pb.d0m_lo = (smax > 0.0f) ? qq * (smax * smax) : 0.0f;
-  pb.negt_hi = (un > rho) ? qn * ((smax > 0.0f) ? fminf(-cmax, 1.0f) : 1.0f) / (un - rho) : INFINITY;
   if (!(pb.sd_lo == pb.sd_lo)) pb.unbounded = true;  // NaN anywhere -> keep
   pb.front = pb.front && !pb.unbounded;
   return pb;
 }
 
+// Lower bound of t* on the tile for a pair on the q side everywhere (plus), else 0:
+// |q.u|/|u| >= sqrt(qq - sd_hi^2), |u| <= |u_c| + rho.
+__device__ __forceinline__ float pair_t_lb(const PairBound& pb) {
+  return pb.plus ? sqrtf(fmaxf(fmaf(-pb.sd_hi, pb.sd_hi, pb.qq), 0.0f)) / (sqrtf(pb.uu) + sqrtf(pb.ap2)) : 0.0f;
+}
+
+// Upper bound of t* on the tile: q.u/|u| <= min(|q|, q.u_c/|u_c| + |q| tan(angle)),
+// |u| >= |u_c| - rho.
+__device__ __forceinline__ float pair_t_ub(const PairBound& pb) {
+  const float un = sqrtf(pb.uu), rho = sqrtf(pb.ap2);
+  return (un > rho) ? fminf(pb.qn, pb.qu / un + pb.dth) / (un - rho) : INFINITY;
+}
+
 // Upper bound (log2 units) of the depth logit of a pair on the tile rays where it is masked
-// (t* <= 0), or -inf where it cannot be masked: the smaller of two bounds.
+// (t* <= 0), or -inf where it cannot be masked: the smaller of two bounds.  The first is
+// the unmasked one with |q.u|/|u| <= sqrt(qq - d0_lo) on the tile; the second holds on
+// the masked part of the tile cone (angle to q in [90 deg, theta_c + phi]), from the
+// cos/sin of theta_c and of the half-angle phi: sin^2 >= sin^2(theta_c + phi) (0 past 180
+// deg) and -cos <= min(-cos(theta_c + phi), 1).
 __device__ __forceinline__ float masked_logit_ub(const PairBound& pb, float w, float b3l, float b2l) {
+  const float un = sqrtf(pb.uu), rho = sqrtf(pb.ap2);
+  const float t_any_lb = (un > rho) ? -(sqrtf(fmaxf(fmaf(-pb.sd_lo, pb.sd_lo, pb.qq), 0.0f)) / (un - rho)) : -INFINITY;
+  const float cth = pb.qu / (pb.qn * un);
+  const float sth = pb.sdc / pb.qn;
+  const float cph = 1.0f / sqrtf(fmaf(pb.tanm, pb.tanm, 1.0f));
+  const float sph = pb.tanm * cph;
+  const float cmax = fmaf(cth, cph, -(sth * sph));
+  const float smax = fmaf(sth, cph, cth * sph);
+  const bool can_mask = !(cmax > 0.0f);
+  const float d0m_lo = (smax > 0.0f) ? pb.qq * (smax * smax) : 0.0f;
+  const float negt_hi = (un > rho) ? pb.qn * ((smax > 0.0f) ? fminf(-cmax, 1.0f) : 1.0f) / (un - rho) : INFINITY;
   const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
-  const float u1 = fmaf(b3l, dlo, -(b2l * pb.t_any_lb));
-  const float u2 = pb.can_mask ? fmaf(b3l, fmaf(-0.5f, pb.d0m_lo, w), b2l * pb.negt_hi) : -INFINITY;
+  const float u1 = fmaf(b3l, dlo, -(b2l * t_any_lb));
+  const float u2 = can_mask ? fmaf(b3l, fmaf(-0.5f, d0m_lo, w), b2l * negt_hi) : -INFINITY;
   return fminf(u1, u2);
+}
+
+// Phase-2 keep test of a bounded pair: the depth-logit bound b3l (w - sd_lo^2/2) - b2l t_lb
+// reaches tau.  With b2l >= 0 and t_lb >= 0 the fma never exceeds the rounded b3l dlo, so
+// a pair whose b3l dlo is below tau fails the full test too and t_lb is not needed.
+__device__ __forceinline__ bool depth_keep(const PairBound& pb, float w, float b3l, float b2l, float tau) {
+  const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
+  if (b2l >= 0.0f && !(b3l * dlo >= tau)) return false;
+  return fmaf(b3l, dlo, -(b2l * pair_t_lb(pb))) >= tau;
 }
 
 }  // namespace fe

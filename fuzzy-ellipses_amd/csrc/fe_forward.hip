@@ -363,7 +363,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
         if (__float_as_int(r[15]) & 1) continue;
         const PairBound pb = pair_bound(r, tg[k]);
         if (pb.plus && !pb.unbounded)
-          lb = fmaxf(lb, fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pb.t_ub)));
+          lb = fmaxf(lb, fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
       }
       for (int o = 32; o > 0; o >>= 1) lb = fmaxf(lb, __shfl_xor(lb, o, 64));
       if (lane == 0) sh_lb[wave] = lb;
@@ -434,9 +434,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
         if (PHASE == 1) {
           keep_s = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
         } else {
-          const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
-          const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
-          keep_s = pb.unbounded || (ub >= tau_s);
+          keep_s = pb.unbounded || depth_keep(pb, w, b3l, b2l, tau_s);
           plus_s = pb.plus;
           if (!pb.plus) ubm_s = masked_logit_ub(pb, w, b3l, b2l);
         }
@@ -515,9 +513,7 @@ __global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs,
           if (PHASE == 1) {
             keep = pb.unbounded || (fmaf(0.5f * pb.sd_lo, pb.sd_lo, -w) <= jb.c_alpha);
           } else {
-            const float dlo = fmaf(-0.5f * pb.sd_lo, pb.sd_lo, w);
-            const float ub = fmaf(b3l, dlo, -(b2l * pb.t_lb));
-            keep = pb.unbounded || (ub >= tau[k]);
+            keep = pb.unbounded || depth_keep(pb, w, b3l, b2l, tau[k]);
             if (!pb.plus) ubm[k] = fmaxf(ubm[k], masked_logit_ub(pb, w, b3l, b2l));
           }
           if (keep) {
