@@ -270,7 +270,7 @@ __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4*
     }
     const float Gq = gd2 * qq;
     const float h = fmaf(Gq, X, g1 + g1);
-    const float gX = fmaf(-(t * t), h, fmaf(Gq, t, g1 * rr));
+    const float gX = rr * fmaf(h, Wr, -g1);   // = g1 rr + Gq t - t^2 h (1 - X^2 rr = W rr)
     const float c = h * (t * rr);
     const float c1 = c * Y1, c2 = c * Y2;
     G[0] = fmaf(gX, xf, G[0]); G[1] = fmaf(gX, yf, G[1]); G[2] += gX;
@@ -348,7 +348,7 @@ __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* p
       }
       const float Gq = gd2 * qq;
       const float h = fmaf(Gq, X, g1 + g1);
-      const float gX = fmaf(-(t * t), h, fmaf(Gq, t, g1 * rr));
+      const float gX = rr * fmaf(h, Wr, -g1);   // = g1 rr + Gq t - t^2 h (1 - X^2 rr = W rr)
       const float c = h * (t * rr);
       SgXx = fmaf(gX, xl, SgXx); Rg += gX;
       Cxx = fmaf(c, xl2, Cxx); Rcx = fmaf(c, xl, Rcx); Rc += c;
