@@ -292,11 +292,13 @@ __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4*
 // The argmax variant (tiles where the 1e-20 floor is visible) evaluates X, Y1, Y2 at the
 // global coordinates with the forward's exact operations, so that the pair holding the
 // maximum logit is found by equality; elsewhere they use the tile-local form.
-template <bool MASK, bool ARGMAX>
+// HALVES: the two half-waves take consecutive row blocks of the range (lanes 32-63 are
+// shifted down by roff rows) for the same 32 Gaussians; the caller adds the halves' sums.
+template <bool MASK, bool ARGMAX, bool HALVES>
 __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* pxb, const float2* pxc, int p0,
-                                              int p1, int wv, float hc, int px0, int py0, float cx, float cy,
-                                              const float* k, float wL, float cG, float qq, float b3, float b2,
-                                              float kap, float* G) {
+                                              int p1, int roff, int wv, float hc, int px0, int py0, float cx,
+                                              float cy, const float* k, float wL, float cG, float qq, float b3,
+                                              float b2, float kap, float* G) {
   const float kx0 = k[0], kx1 = k[1], k10 = k[3], k11 = k[4], k20 = k[6], k21 = k[7];
   // global coordinates of the local origin, and the forms' constants there
   const float ox = ((float)px0 + hc) - cx, oy = ((float)py0 + hc) - cy;
@@ -307,17 +309,19 @@ __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* p
   float SgXx = 0.f, SgXy = 0.f, SgX = 0.f;
   float Cxx = 0.f, Cxy = 0.f, Cyy = 0.f, Cx = 0.f, Cy = 0.f, C1 = 0.f;
   float S9 = 0.f, S10 = 0.f;
+  const int ro = HALVES ? roff : 0;   // this lane's row offset
+  const int poff = ro * wv;
   int p = p0;
   while (p < p1) {
     const int iy = p / wv;
     const int pe = min(p1, (iy + 1) * wv);
-    const float yl = (float)iy - hc;
-    const float yr = ARGMAX ? (float)(py0 + iy) - cy : yl;
+    const float yl = (float)(iy + ro) - hc;
+    const float yr = ARGMAX ? (float)(py0 + iy + ro) - cy : yl;
     const float XR = fmaf(kx1, yr, kxr), Y1R = fmaf(k11, yr, k1r), Y2R = fmaf(k21, yr, k2r);
     float Rg = 0.f, Rcx = 0.f, Rc = 0.f;
     for (; p < pe; ++p) {
-      const float4 pa = pxa[p];
-      const float4 pb = pxb[p];
+      const float4 pa = pxa[p + poff];
+      const float4 pb = pxb[p + poff];
       const float xl = pa.x, xl2 = pa.y, M = pa.z, cE = pa.w;
       const float A1 = pb.x, A2 = pb.y, cDb = pb.z, B1 = pb.w;
       const float xr = ARGMAX ? (float)(px0 + (p - iy * wv)) - cx : xl;
@@ -335,7 +339,7 @@ __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* p
       if (MASK) s = (t > 0.0f) ? s : 0.0f;
       float g1, gd2;
       if (ARGMAX) {
-        const float2 pc = pxc[p];
+        const float2 pc = pxc[p + poff];
         g1 = fmaf(s, fmaf(-A2, t, A1), pc.x);                  // dL/dt
         gd2 = fmaf(s, fmaf(cDb, t, B1), cE * e);                // dL/dd2
         const float ex = (b3 * lt == M) ? pc.y : 0.0f;          // the pair holding the maximum logit
@@ -452,10 +456,15 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     // The next chunk's records are loaded before this chunk's atomics are issued: stores
     // and atomics count in the same wait counter as loads, and waiting for records loaded
     // after the atomics would wait for the atomics to complete at memory.
+    // A last chunk of at most 32 Gaussians on a full pixel-grid tile runs on half-waves: lanes
+    // l and l + 32 take the same Gaussian over the two halves of the wave's rows.
+    const bool can_halve = rows && npx == TS * TS;
+    const int half_rows = (q / 2) / wv;
+    auto halved = [&](int64_t c0) { return can_halve && total - c0 <= 32; };
     int64_t gid_n = 0;
     float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa, qc = qa;
     auto fetch = [&](int64_t c0) {
-      const int64_t kk = c0 + lane;
+      const int64_t kk = c0 + (halved(c0) ? (lane & 31) : lane);
       if (kk < total) {
         gid_n = A.dense ? kk : (int64_t)A.list[kk < n1 ? s1 + kk : s2 + (kk - n1)];
         qa = rec24[gid_n * 4 + 0];
@@ -467,7 +476,8 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     asm volatile("" ::"v"(qa.x), "v"(qa.y), "v"(qa.z), "v"(qa.w), "v"(qb.x), "v"(qb.y), "v"(qb.z), "v"(qb.w),
                  "v"(qc.x), "v"(qc.y), "v"(qc.z), "v"(qc.w));
     for (int64_t c0 = 0; c0 < total; c0 += 64) {
-      const bool act = c0 + lane < total;
+      const bool hv = halved(c0);   // block-uniform
+      const bool act = c0 + (hv ? (lane & 31) : lane) < total;
       const int64_t gid = gid_n;
       const float4 q0 = qa, q1 = qb, q2 = qc;
       float G[11];
@@ -475,23 +485,40 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
       for (int k = 0; k < 11; ++k) G[k] = 0.0f;
       if (act) {
         const float k9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
-        if (rows) {
+        if (rows && hv) {
+          const int px0 = tx * TS, py0 = ty * TS;
+          const int pe = p0 + q / 2, ro = (lane >= 32) ? half_rows : 0;
+          if (masked && floor_tile)
+            bwd_rows_rec2<true, true, true>(pxa, pxb, pxc, p0, pe, ro, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+                                            q2.z, q2.w, b3, b2, kap, G);
+          else if (masked)
+            bwd_rows_rec2<true, false, true>(pxa, pxb, pxc, p0, pe, ro, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+                                             q2.z, q2.w, b3, b2, kap, G);
+          else
+            bwd_rows_rec2<false, false, true>(pxa, pxb, pxc, p0, pe, ro, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+                                              q2.z, q2.w, b3, b2, kap, G);
+        } else if (rows) {
           const int px0 = tx * TS, py0 = ty * TS;
           if (masked && floor_tile)
-            bwd_rows_rec2<true, true>(pxa, pxb, pxc, p0, p1, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y, q2.z, q2.w,
-                                      b3, b2, kap, G);
+            bwd_rows_rec2<true, true, false>(pxa, pxb, pxc, p0, p1, 0, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+                                             q2.z, q2.w, b3, b2, kap, G);
           else if (masked)
-            bwd_rows_rec2<true, false>(pxa, pxb, pxc, p0, p1, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y, q2.z, q2.w,
-                                       b3, b2, kap, G);
+            bwd_rows_rec2<true, false, false>(pxa, pxb, pxc, p0, p1, 0, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+                                              q2.z, q2.w, b3, b2, kap, G);
           else
-            bwd_rows_rec2<false, false>(pxa, pxb, pxc, p0, p1, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y, q2.z,
-                                        q2.w, b3, b2, kap, G);
+            bwd_rows_rec2<false, false, false>(pxa, pxb, pxc, p0, p1, 0, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+                                               q2.z, q2.w, b3, b2, kap, G);
         } else {
           if (masked && floor_tile)
             bwd_pixels_rec2<true, true>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
           else if (masked) bwd_pixels_rec2<true, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
           else bwd_pixels_rec2<false, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
         }
+      }
+      if (hv) {
+        // add the other half-wave's sums (same Gaussian in lanes l and l + 32)
+#pragma unroll
+        for (int k = 0; k < 11; ++k) G[k] += __shfl_xor(G[k], 32, 64);
       }
 #pragma unroll
       for (int k = 0; k < 11; ++k) red[wave][k][lane] = G[k];
@@ -510,7 +537,7 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         const int k = wave + 4 * i;
-        if (act && k < 11) atomicAdd(gsum + (int64_t)k * A.total_rec + gid, v[i]);
+        if (act && !(hv && lane >= 32) && k < 11) atomicAdd(gsum + (int64_t)k * A.total_rec + gid, v[i]);
       }
       __syncthreads();
     }
