@@ -219,11 +219,13 @@ class Renderer:
     def debug_state(self) -> dict:
         """Host copies of the last forward's workspace internals (tests only): per-Gaussian
         records and their order (record j of a view is Gaussian order[j]), tile ranges and
-        lists of both phases (record ids), per-tile phase-1 thresholds and floor seeds."""
+        lists of both phases (record ids), per-tile phase-1 thresholds and floor seeds, and the
+        tile dispatch orders of the phase-1 blend and of the backward (2, T)."""
         torch.cuda.synchronize(self.device)
         ws = self._ws.cpu().numpy()
-        h = ws[:8 * 13].view(np.uint64)
-        magic, o_cnt, o_off1, o_off2, T, mode, o_rec, o_list, o_lmin, n_rec, o_st, n_pix, o_ord = [int(x) for x in h]
+        h = ws[:8 * 14].view(np.uint64)
+        magic, o_cnt, o_off1, o_off2, T, mode, o_rec, o_list, o_lmin, n_rec, o_st, n_pix, o_ord, o_tord = \
+            [int(x) for x in h]
         out = {"records": ws[o_rec:o_rec + 64 * n_rec].view(np.float32).reshape(n_rec, 16), "mode": mode,
                "order": ws[o_ord:o_ord + 4 * n_rec].view(np.int32).copy()}
         if mode == _lib.FE_MODE_CULLED:
@@ -232,7 +234,8 @@ class Renderer:
             total = int(off2[-1])
             out.update(off1=off1, off2=off2, list=ws[o_list:o_list + 4 * total].view(np.uint32).copy(),
                        tile_lmin=ws[o_lmin:o_lmin + 4 * T].view(np.float32).copy(),
-                       tile_seed=ws[o_lmin + 4 * T:o_lmin + 8 * T].view(np.float32).copy())
+                       tile_seed=ws[o_lmin + 4 * T:o_lmin + 8 * T].view(np.float32).copy(),
+                       tile_order=ws[o_tord:o_tord + 8 * T].view(np.int32).reshape(2, T).copy())
         return out
 
     # -- backward -----------------------------------------------------------------------

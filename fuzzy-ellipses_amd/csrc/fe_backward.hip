@@ -189,6 +189,7 @@ struct BwdArgs {
   float* gsum;          // the 11 blend-record sums per (Gaussian, view) record, one plane each
   int64_t total_rec;    // plane stride
   int dense;
+  const int32_t* tord;  // dispatch order (block b takes tile tord[b]), or null
 };
 
 // Sums per record.  Planes, not records of 11 floats: device-scope float atomics to one
@@ -240,7 +241,7 @@ __device__ __forceinline__ PixTerms pix_terms(const BwdArgs& A, const Job& jb, c
 // G[10] = sum g_d2 W r.
 template <bool MASK, bool ARGMAX>
 __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4* pxb, const float2* pxc, int p0,
-                                                int p1, const float* k, float wL, float cG, float qq, float b3,
+                                                int p1, const float* k, float wL, float ncG, float qq, float b3,
                                                 float b2, float kap, float* G) {
   const float kx0 = k[0], kx1 = k[1], kx2 = k[2], k10 = k[3], k11 = k[4], k12 = k[5], k20 = k[6], k21 = k[7],
               k22 = k[8];
@@ -255,7 +256,7 @@ __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4*
     const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
     const float t = X * rr;
     const float Wr = W * rr;
-    const float d2l = fmaf(-cG, Wr, wL);
+    const float d2l = fmaf(ncG, Wr, wL);
     const float e = __builtin_amdgcn_exp2f(d2l);
     const float lt = fmaf(kap, t, d2l);
     float s = __builtin_amdgcn_exp2f(fmaf(b3, lt, -M));
@@ -297,7 +298,7 @@ __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4*
 template <bool MASK, bool ARGMAX, bool HALVES>
 __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* pxb, const float2* pxc, int p0,
                                               int p1, int roff, int wv, float hc, int px0, int py0, float cx,
-                                              float cy, const float* k, float wL, float cG, float qq, float b3,
+                                              float cy, const float* k, float wL, float ncG, float qq, float b3,
                                               float b2, float kap, float* G) {
   const float kx0 = k[0], kx1 = k[1], k10 = k[3], k11 = k[4], k20 = k[6], k21 = k[7];
   // global coordinates of the local origin, and the forms' constants there
@@ -332,7 +333,7 @@ __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* p
       const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
       const float t = X * rr;
       const float Wr = W * rr;
-      const float d2l = fmaf(-cG, Wr, wL);
+      const float d2l = fmaf(ncG, Wr, wL);
       const float e = __builtin_amdgcn_exp2f(d2l);
       const float lt = fmaf(kap, t, d2l);
       float s = __builtin_amdgcn_exp2f(fmaf(b3, lt, -M));
@@ -399,7 +400,8 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   __shared__ float2 pxc[NPIX];   // 1e-20 cD, cX (maximum-logit term)
   __shared__ float red[4][11][64];
 
-  const int64_t gt = xcd_tile(blockIdx.x, gridDim.x);   // one block per tile
+  // one block per tile
+  const int64_t gt = A.tord ? (int64_t)A.tord[blockIdx.x] : xcd_tile(blockIdx.x, gridDim.x);
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
   const Job& jb = A.jobs[ji];
   const BwdJob& bj = A.bjobs[ji];
@@ -651,13 +653,13 @@ hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Jo
                             int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                             const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
                             const float* st, int64_t total_pix, const int32_t* job_flags, float* gsum,
-                            int64_t total_rec, int dense) {
+                            int64_t total_rec, int dense, const int32_t* tord) {
   const int64_t n4 = (total_rec * kGsumPlanes + 3) / 4;   // the region is padded to 256 B
   if (n4 > 0)
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((n4 + 255) / 256, 4096)), dim3(256), 0, s,
                        reinterpret_cast<float4*>(gsum), n4);
   BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, rec2, tile_np, off1, off2, list, st,
-            total_pix, job_flags, gsum, total_rec, dense};
+            total_pix, job_flags, gsum, total_rec, dense, tord};
   hipError_t e = hipErrorInvalidValue;
   switch (ts) {
     case 8: e = bwd_t<8>(s, total_tiles, a); break;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -49,7 +50,7 @@ struct Layout {
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_rec2 = 0, o_tnp = 0,
          o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
-         o_bjobs = 0, o_gsum = 0, o_rect = 0, o_order = 0, o_rank = 0, bytes = 0;
+         o_bjobs = 0, o_gsum = 0, o_rect = 0, o_order = 0, o_rank = 0, o_tord = 0, bytes = 0;
 };
 
 // A small header at the start of the workspace lets fe_read_stats find the counters.
@@ -57,7 +58,7 @@ struct WsHeader {
   uint64_t magic;
   uint64_t o_counters, o_off1, o_off2, total_tiles, mode;
   uint64_t o_rec, o_list, o_lmin, total_rec, o_state, total_pix;
-  uint64_t o_order;
+  uint64_t o_order, o_tord;
 };
 constexpr uint64_t kMagic = 0x4645414d44313031ull;  // "FEAMD101"
 
@@ -209,6 +210,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     L->o_off1 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
     L->o_off2 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
     L->o_partial = p; p = align256(p + sizeof(unsigned long long) * (scan_partial_count(tiles) + 1));
+    L->o_tord = p; p = align256(p + sizeof(int32_t) * 2 * tiles);   // phase-1 order, then union order
     L->o_list = p; p = align256(p + sizeof(uint32_t) * (size_t)std::max<int64_t>(cap, 1));
   }
   if (L->save) {
@@ -338,7 +340,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   std::vector<char> blob(L.o_counters);
   WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode,
              L.o_rec, L.o_list, L.o_lmin, (uint64_t)L.total_rec, L.o_state, (uint64_t)L.total_pix,
-             L.o_order};
+             L.o_order, L.o_tord};
   memcpy(blob.data() + L.o_header, &h, sizeof(h));
   memcpy(blob.data() + L.o_jobs, jobs.data(), sizeof(Job) * n_views);
   if (!L.reuse) {
@@ -367,7 +369,7 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
       return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, rec2, nullptr, nullptr, nullptr,
-                          nullptr, state, L.total_pix, lmin, seed, ubm, cnt, flags, L.save);
+                          nullptr, state, L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
     }));
     return FE_OK;
   }
@@ -379,16 +381,20 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   unsigned long long* partial = at<unsigned long long>(ws, L.o_partial);
   uint32_t* list = at<uint32_t>(ws, L.o_list);
   uint32_t* tnp = at<uint32_t>(ws, L.o_tnp);
+  int32_t* tord = at<int32_t>(ws, L.o_tord);   // [0, T): phase-1 blend order; [T, 2T): backward order
+  uint32_t* thist = at<uint32_t>(ws, L.o_counters + offsetof(Counters, tile_hist));
   // phase 1
   FE_HIP(timed(s, FE_K_MASK1, [&] {
     return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, seed, ubm, tnp,
                             flags, cnt);
   }));
-  FE_HIP(timed(s, FE_K_SCAN1, [&] { return launch_scan(s, count, L.total_tiles, partial, off1, nullptr); }));
+  FE_HIP(timed(s, FE_K_SCAN1, [&] {
+    return launch_scan(s, count, L.total_tiles, partial, off1, nullptr, nullptr, thist, tord);
+  }));
   FE_HIP(timed(s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }));
   FE_HIP(timed(s, FE_K_BLEND1, [&] {
     return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
-                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save);
+                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save, tord);
   }));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
@@ -396,12 +402,14 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
                             flags, cnt);
   }));
   FE_HIP(timed(s, FE_K_SCAN2, [&] {
-    return launch_scan(s, count, L.total_tiles, partial, off2, off1 + L.total_tiles);
+    // the backward's order: both lists of a tile (phase-1 length from off1)
+    return launch_scan(s, count, L.total_tiles, partial, off2, off1 + L.total_tiles, off1,
+                       thist + kTileBuckets, tord + L.total_tiles);
   }));
   FE_HIP(timed(s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }));
   FE_HIP(timed(s, FE_K_BLEND2, [&] {
     return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
-                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save);
+                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
   }));
   return FE_OK;
 }
@@ -446,7 +454,8 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off1),
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off2),
                             dense ? nullptr : at<uint32_t>(ws, L.o_list), at<float>(ws, L.o_state), L.total_pix,
-                            at<int32_t>(ws, L.o_flags), at<float>(ws, L.o_gsum), L.total_rec, dense ? 1 : 0);
+                            at<int32_t>(ws, L.o_flags), at<float>(ws, L.o_gsum), L.total_rec, dense ? 1 : 0,
+                            dense ? nullptr : at<int32_t>(ws, L.o_tord) + L.total_tiles);
   }));
   return FE_OK;
 }

@@ -18,7 +18,7 @@
 //   [3..5]  kY1 = (a.e1, b.e1, g.e1) / |q|
 //   [6..8]  kY2 = (a.e2, b.e2, g.e2) / |q|
 //   [9]     w*log2(e)
-//   [10]    |q|^2 log2(e) / 2
+//   [10]    -|q|^2 log2(e) / 2   (stored negated: d2 log2(e) = [9] + [10] W/den is one fma)
 //   [11]    |q|^2
 //   [12..15] unused
 // where (q/|q|, e1, e2) is an orthonormal basis.  For p = (x-cx, y-cy, 1) the linear forms
@@ -113,6 +113,20 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t n) {
   return x * per + (x < rem ? x : rem) + k;
 }
 
+// Dispatch order of the list-driven blends (the culled phase-1 blend and the backward).
+// Blocks are dispatched in index order and a tile's time grows with its list length, so in
+// spatial order the longest tiles can start in the last wave of blocks and leave the chip
+// idle behind them.  The scan of the tile counts also sorts the tiles by a bucket of their
+// count, longest first (longest-processing-time-first scheduling), and the blends map block
+// b to tile tord[b].  The bucket is a float-like code of the count with 3 mantissa bits
+// (monotone, within 12.5%), so the order is a counting sort over kTileBuckets buckets.
+constexpr int kTileBuckets = 256;
+__device__ __forceinline__ int tile_bucket(uint32_t c) {
+  if (c < 8u) return (int)c;
+  const int e = 31 - __clz(c);
+  return 8 * (e - 2) + (int)((c >> (e - 3)) & 7u);   // at most 8 * 29 + 7 = 239
+}
+
 // Scene index of record j of a view.
 __device__ __forceinline__ int64_t gauss_of(const Job& jb, int64_t j) { return jb.order ? jb.order[j] : j; }
 
@@ -135,6 +149,7 @@ struct Counters {
   unsigned long long floor_unresolved;  // pixels' tiles where it still could (must stay 0)
   unsigned long long pad[14];
   unsigned long long slot[kCountSlots][16];   // [i][0]: evaluated pairs, [i][1]: of them phase 1
+  uint32_t tile_hist[2][kTileBuckets];   // dispatch buckets: tiles per bucket, then bucket starts
 };
 
 // Whether a view's blends use the blend records: no degenerate record, every record valid,

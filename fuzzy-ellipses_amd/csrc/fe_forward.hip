@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
       o2[6 + c] = (float)(m2 * iq);
     }
     o2[9] = (float)((double)w * 1.4426950408889634);
-    o2[10] = (float)(qq * 0.7213475204444817);
+    o2[10] = (float)(qq * -0.7213475204444817);
     o2[11] = (float)qq;
 #pragma unroll
     for (int i = 0; i < 12; ++i) ok = ok && isfinite(o2[i]);
@@ -654,23 +654,44 @@ __device__ __forceinline__ unsigned long long block_excl_scan256(unsigned long l
   return wbase + inc - v;
 }
 
+// Dispatch key of tile i: its list length, plus its phase-1 length when key_off (the
+// phase-1 offsets) is given, so that the backward orders tiles by both lists.
+__device__ __forceinline__ uint32_t tile_key(const uint32_t* cnt, const unsigned long long* key_off, int64_t i) {
+  return cnt[i] + (key_off ? (uint32_t)(key_off[i + 1] - key_off[i]) : 0u);
+}
+
+// With hist, each block also adds its tiles' dispatch-bucket histogram into hist.
 __global__ __launch_bounds__(256) void k_scan_reduce(const uint32_t* __restrict__ cnt, int64_t n,
-                                                     unsigned long long* __restrict__ partial) {
+                                                     unsigned long long* __restrict__ partial,
+                                                     const unsigned long long* __restrict__ key_off,
+                                                     uint32_t* __restrict__ hist) {
+  static_assert(kTileBuckets == 256, "one bucket per thread");
   __shared__ unsigned long long sh[4];
+  __shared__ uint32_t lh[kTileBuckets];
+  if (hist) {   // grid-uniform
+    lh[threadIdx.x] = 0u;
+    __syncthreads();
+  }
   const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
   unsigned long long s = 0;
   for (int k = 0; k < 4; ++k) {
     const int64_t i = b0 + threadIdx.x * 4 + k;
-    if (i < n) s += cnt[i];
+    if (i < n) {
+      s += cnt[i];
+      if (hist) atomicAdd(&lh[tile_bucket(tile_key(cnt, key_off, i))], 1u);
+    }
   }
   unsigned long long tot;
-  block_excl_scan256(s, sh, &tot);
+  block_excl_scan256(s, sh, &tot);   // ends with a barrier
   if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+  if (hist && lh[threadIdx.x] != 0u) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
 }
 
-// single block: partial[i] <- base + exclusive prefix of partial
+// single block: partial[i] <- base + exclusive prefix of partial; with hist, the bucket
+// counts become the buckets' first positions, largest bucket first
 __global__ __launch_bounds__(256) void k_scan_top(unsigned long long* __restrict__ partial, int64_t nb,
-                                                  const unsigned long long* __restrict__ base_src) {
+                                                  const unsigned long long* __restrict__ base_src,
+                                                  uint32_t* __restrict__ hist) {
   __shared__ unsigned long long sh[4];
   unsigned long long carry = base_src ? *base_src : 0ull;
   for (int64_t c = 0; c < nb; c += 256) {
@@ -681,27 +702,60 @@ __global__ __launch_bounds__(256) void k_scan_top(unsigned long long* __restrict
     if (i < nb) partial[i] = carry + ex;
     carry += tot;
   }
+  if (hist) {
+    const int b = kTileBuckets - 1 - (int)threadIdx.x;
+    unsigned long long tot;
+    const unsigned long long ex = block_excl_scan256(hist[b], sh, &tot);
+    hist[b] = (uint32_t)ex;
+  }
 }
 
+// With tord, tile i is also placed in the dispatch order: a block ranks its tiles within
+// each bucket in LDS, reserves a run of the bucket's positions with one atomic, and writes
+// tord[position] = i.  (Which run a block gets varies from call to call; every tile's
+// result is independent of the order in which the tiles run.)
 __global__ __launch_bounds__(256) void k_scan_final(const uint32_t* __restrict__ cnt, int64_t n,
                                                     const unsigned long long* __restrict__ partial,
-                                                    unsigned long long* __restrict__ off) {
+                                                    unsigned long long* __restrict__ off,
+                                                    const unsigned long long* __restrict__ key_off,
+                                                    uint32_t* __restrict__ hist, int32_t* __restrict__ tord) {
   __shared__ unsigned long long sh[4];
+  __shared__ uint32_t lh[kTileBuckets], lb[kTileBuckets];
+  if (tord) {   // grid-uniform
+    lh[threadIdx.x] = 0u;
+    __syncthreads();
+  }
   const int64_t b0 = (int64_t)blockIdx.x * kScanBlock;
-  uint32_t v[4];
+  uint32_t v[4], rk[4];
+  int bk[4];
   unsigned long long s = 0;
   for (int k = 0; k < 4; ++k) {
     const int64_t i = b0 + threadIdx.x * 4 + k;
     v[k] = (i < n) ? cnt[i] : 0u;
     s += v[k];
+    bk[k] = 0;
+    rk[k] = 0u;
+    if (tord && i < n) {
+      bk[k] = tile_bucket(tile_key(cnt, key_off, i));
+      rk[k] = atomicAdd(&lh[bk[k]], 1u);
+    }
   }
   unsigned long long tot;
-  unsigned long long ex = block_excl_scan256(s, sh, &tot) + partial[blockIdx.x];
+  unsigned long long ex = block_excl_scan256(s, sh, &tot) + partial[blockIdx.x];   // ends with a barrier
   for (int k = 0; k < 4; ++k) {
     const int64_t i = b0 + threadIdx.x * 4 + k;
     if (i < n) off[i] = ex;
     ex += v[k];
     if (i == n - 1) off[n] = ex;
+  }
+  if (tord) {
+    const uint32_t c = lh[threadIdx.x];
+    if (c != 0u) lb[threadIdx.x] = atomicAdd(&hist[threadIdx.x], c);
+    __syncthreads();
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = b0 + threadIdx.x * 4 + k;
+      if (i < n) tord[lb[bk[k]] + rk[k]] = (int32_t)i;
+    }
   }
 }
 
@@ -851,7 +905,7 @@ __device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, con
       const float4 r0 = lds[(k0 + s) * 3 + 0];
       const float4 r1 = lds[(k0 + s) * 3 + 1];
       const float4 r2 = lds[(k0 + s) * 3 + 2];
-      // kX = r0.xyz, kY1 = (r0.w, r1.x, r1.y), kY2 = (r1.z, r1.w, r2.x), wL = r2.y, cG = r2.z
+      // kX = r0.xyz, kY1 = (r0.w, r1.x, r1.y), kY2 = (r1.z, r1.w, r2.x), wL = r2.y, -cG = r2.z
       // ROW: the thread's pixels share y, so the forms' y-parts are computed once (the same
       // operations as per pixel, so the values are identical)
       const float RX = ROW ? fmaf(r0.y, yf[0], r0.z) : 0.0f;
@@ -867,7 +921,7 @@ __device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, con
         const float t = X * rr;
         // log2(e) * d2, rounded exactly as the backward recomputes it (it finds the pair
         // holding the maximum logit by equality)
-        const float d2l = fmaf(-r2.z, W * rr, r2.y);
+        const float d2l = fmaf(r2.z, W * rr, r2.y);
         E[q] += __builtin_amdgcn_exp2f(d2l);
         const float lt = fmaf(kap, t, d2l);               // log2(e) * (b3 d2 - b2 t) / b3
         Zall[q] += t;
@@ -917,6 +971,7 @@ struct BlendArgs {
   Counters* cnt;
   const int32_t* job_flags;
   int save;
+  const int32_t* tord;  // dispatch order (block b renders tile tord[b]), or null
 };
 
 // Pixels per thread of the blend: two for 16- and 32-pixel tiles (each record read from LDS
@@ -936,7 +991,8 @@ __global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
   __shared__ float4 lds[GB * 4];
   __shared__ float sh_red[NT / 64 + 1];
 
-  const int64_t gt = xcd_tile(blockIdx.x, gridDim.x);   // one block per tile
+  // one block per tile
+  const int64_t gt = A.tord ? (int64_t)A.tord[blockIdx.x] : xcd_tile(blockIdx.x, gridDim.x);
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
   const Job& jb = A.jobs[ji];
   const int tile = (int)(gt - jb.tile_base);
@@ -1208,11 +1264,13 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
 int64_t scan_partial_count(int64_t n) { return (n + kScanBlock - 1) / kScanBlock; }
 
 hipError_t launch_scan(hipStream_t s, const uint32_t* cnt, int64_t n, unsigned long long* partial,
-                       unsigned long long* off, const unsigned long long* base_src) {
+                       unsigned long long* off, const unsigned long long* base_src,
+                       const unsigned long long* key_off, uint32_t* hist, int32_t* tord) {
+  if ((hist == nullptr) != (tord == nullptr)) return hipErrorInvalidValue;
   const int64_t nb = scan_partial_count(n);
-  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, s, cnt, n, partial);
-  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, partial, nb, base_src);
-  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(256), 0, s, cnt, n, partial, off);
+  hipLaunchKernelGGL(k_scan_reduce, dim3((unsigned)nb), dim3(256), 0, s, cnt, n, partial, key_off, hist);
+  hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(256), 0, s, partial, nb, base_src, hist);
+  hipLaunchKernelGGL(k_scan_final, dim3((unsigned)nb), dim3(256), 0, s, cnt, n, partial, off, key_off, hist, tord);
   return hipGetLastError();
 }
 
@@ -1249,9 +1307,9 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
                         const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
                         float* tile_lmin, float* tile_seed, const float* tile_ubm, Counters* cnt,
-                        const int32_t* job_flags, int save) {
+                        const int32_t* job_flags, int save, const int32_t* tord) {
   BlendArgs a{jobs, n_jobs, rec, rec2, tile_np, off1, off2, list, st, total_pix, tile_lmin, tile_seed, tile_ubm, cnt,
-              job_flags, save};
+              job_flags, save, tord};
   switch (ts) {
     case 8: return launch_blend_t<8>(s, gb, phase, total_tiles, a);
     case 16: return launch_blend_t<16>(s, gb, phase, total_tiles, a);

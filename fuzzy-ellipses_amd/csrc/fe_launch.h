@@ -26,8 +26,11 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
                             uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
                             uint32_t* tile_np, const int32_t* job_flags, Counters* cnt);
 int64_t scan_partial_count(int64_t n);
+// With hist and tord (both or neither), also writes the tiles' dispatch order to tord (a
+// counting sort over kTileBuckets buckets of the tile key; hist must start at zero).
 hipError_t launch_scan(hipStream_t s, const uint32_t* cnt, int64_t n, unsigned long long* partial,
-                       unsigned long long* off, const unsigned long long* base_src);
+                       unsigned long long* off, const unsigned long long* base_src,
+                       const unsigned long long* key_off, uint32_t* hist, int32_t* tord);
 hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t total_tiles,
                              const unsigned long long* mask, const unsigned long long* off,
                              uint32_t* list);
@@ -36,13 +39,13 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
                         const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
                         float* tile_lmin, float* tile_seed, const float* tile_ubm, Counters* cnt,
-                        const int32_t* job_flags, int save);
+                        const int32_t* job_flags, int save, const int32_t* tord);
 
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
                             int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                             const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
                             const float* st, int64_t total_pix, const int32_t* job_flags, float* gsum,
-                            int64_t total_rec, int dense);
+                            int64_t total_rec, int dense, const int32_t* tord);
 size_t bwd_gsum_floats();
 
 hipError_t launch_sil_loss(hipStream_t s, const fe_sil_view* views, int n_views, float* cnt_part,

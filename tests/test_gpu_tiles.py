@@ -33,7 +33,21 @@ def _check(fea, scene, cam, ts):
     for t in range(T):
         seg = lst[off1[t]:off1[t + 1]].astype(np.int64)
         assert np.all(np.diff(seg) > 0)
+    _check_order(st)
     return int(off1[-1]), int(off2[-1] - off1[-1])
+
+
+def _check_order(st):
+    """The blends' tile dispatch orders: permutations of the tiles, longest list first up to
+    the bucket resolution (a key may exceed the one before it by at most 1/8 of it, plus 1)."""
+    off1, off2, order = st["off1"], st["off2"], st["tile_order"]
+    T = len(off1) - 1
+    n1 = np.diff(off1)
+    for row, key in ((0, n1), (1, n1 + np.diff(off2))):
+        o = order[row].astype(np.int64)
+        np.testing.assert_array_equal(np.sort(o), np.arange(T))
+        k = key[o].astype(np.float64)
+        assert np.all(k[1:] <= k[:-1] * 1.125 + 1.0), f"order {row} is not longest-first"
 
 
 @pytest.mark.parametrize("ts", [8, 16, 32])
@@ -48,3 +62,16 @@ def test_keys_shell_view(fea, scenes_mod, view):
     scene = scenes_mod.shell_scene()
     k1, k2 = _check(fea, scene, scene.cameras[view], 16)
     print(f"shell view {view}: phase-1 pairs {k1}, phase-2 pairs {k2}")
+
+
+def test_dispatch_order_all_shell_views(fea, scenes_mod):
+    """With 16 views in one call the orders run over every view's tiles."""
+    import torch
+    scene = scenes_mod.shell_scene()
+    r = fea.Renderer(tile=16, batch=64, mode="culled")
+    r.forward([fea.SceneTensors.from_scene(scene)], [(0, c, scene.beta2, scene.beta3) for c in scene.cameras])
+    torch.cuda.synchronize()
+    st = r.debug_state()
+    _check_order(st)
+    k = np.diff(st["off1"])[st["tile_order"][0]]
+    print(f"16 views: {len(k)} tiles, phase-1 lengths in dispatch order {k[:3]} ... {k[-3:]}")
