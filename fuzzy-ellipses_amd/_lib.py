@@ -148,8 +148,18 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     return lib
 
 
-def profile_enable(on: bool) -> None:
-    check(load().fe_profile_enable(1 if on else 0), "fe_profile_enable")
+FE_PROFILE_SELECT = 0x40000000
+
+
+def profile_enable(on: bool, kernels=None) -> None:
+    """Time every kernel launch (or only those named in `kernels`, from KERNELS) from now on."""
+    if on and kernels is not None:
+        bits = 0
+        for k in kernels:
+            bits |= 1 << KERNELS.index(k)
+        check(load().fe_profile_enable(FE_PROFILE_SELECT | bits), "fe_profile_enable")
+    else:
+        check(load().fe_profile_enable(1 if on else 0), "fe_profile_enable")
 
 
 def profile_read(reset: bool = True) -> dict:

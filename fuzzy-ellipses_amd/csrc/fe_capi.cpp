@@ -269,7 +269,7 @@ static fe_status upload(hipStream_t s, void* dst, const void* src, size_t bytes)
 // Optional per-kernel event timing.
 struct Profiler {
   std::mutex mu;
-  bool on = false;
+  uint32_t mask = 0;   // kernels (bit 1 << FE_K_*) whose launches are timed
   std::vector<hipEvent_t> pool;
   struct Rec { int k; hipEvent_t a, b; };
   std::vector<Rec> pending;
@@ -284,10 +284,10 @@ struct Profiler {
 };
 static Profiler g_prof;
 
-// Runs `launch` between two recorded events when profiling is on.
+// Runs `launch` between two recorded events when kernel k is being timed.
 template <typename F>
 static hipError_t timed(hipStream_t s, int k, F launch) {
-  if (!g_prof.on) return launch();
+  if (!((g_prof.mask >> k) & 1u)) return launch();
   hipEvent_t a, b;
   {
     std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -533,7 +533,8 @@ fe_status fe_read_stats(void* stream, const void* workspace, size_t workspace_by
 
 fe_status fe_profile_enable(int32_t on) {
   std::lock_guard<std::mutex> lk(fe::g_prof.mu);
-  fe::g_prof.on = on != 0;
+  constexpr uint32_t all = (1u << FE_K_COUNT) - 1u;
+  fe::g_prof.mask = (on & FE_PROFILE_SELECT) ? ((uint32_t)on & all) : (on != 0 ? all : 0u);
   return FE_OK;
 }
 

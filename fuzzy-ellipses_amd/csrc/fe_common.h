@@ -283,12 +283,13 @@ __device__ __forceinline__ PairBound pair_bound(const float* r, const TileGeo& t
     ap[i] = fmaf(a[i], tg.hx, b[i] * tg.hy);
     am[i] = fmaf(a[i], tg.hx, -(b[i] * tg.hy));
   }
-  const float n1 = sqrtf(dot3(pp, pp));
-  const float n2 = sqrtf(dot3(pm, pm));
+  // larger of |pp|, |pm| with one square root (sqrt is monotone and correctly rounded, so
+  // this equals the larger of the two roots)
+  const float nmax = sqrtf(fmaxf(dot3(pp, pp), dot3(pm, pm)));
   const float den = fmaf(-fabsf(ub), tg.hy, fmaf(-fabsf(ua), tg.hx, uu));
   PairBound pb;
   pb.unbounded = !(den > 0.0f);
-  const float tanm = fmaxf(n1, n2) / den;
+  const float tanm = nmax / den;
   const float qn = sqrtf(qq);
   const float dth = qn * tanm;
   pb.sd_lo = fmaxf(sdc - dth, 0.0f);

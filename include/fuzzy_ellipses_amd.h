@@ -170,11 +170,15 @@ fe_status fe_read_stats(void* stream, const void* workspace, size_t workspace_by
 /* ---- kernel timing (diagnostics) ----
  * When enabled, every kernel launch of fe_forward/fe_backward is bracketed by hipEvents on
  * the caller's stream; fe_profile_read synchronises those events and reports the summed
- * durations and launch counts per kernel, then optionally resets them. */
+ * durations and launch counts per kernel, then optionally resets them.
+ * fe_profile_enable(0) turns timing off, fe_profile_enable(1) times every kernel, and
+ * fe_profile_enable(FE_PROFILE_SELECT | bits) times only the kernels k with (1 << k) set
+ * (each timed launch adds two event records to the stream). */
 enum {
   FE_K_PREPROCESS = 0, FE_K_MASK1, FE_K_SCAN1, FE_K_WRITE1, FE_K_BLEND1, FE_K_MASK2, FE_K_SCAN2,
   FE_K_WRITE2, FE_K_BLEND2, FE_K_BLEND_DENSE, FE_K_BWD_BLEND, FE_K_BWD_FINALIZE, FE_K_COUNT
 };
+#define FE_PROFILE_SELECT 0x40000000
 typedef struct fe_kernel_times {
   double ms[FE_K_COUNT];
   int64_t launches[FE_K_COUNT];

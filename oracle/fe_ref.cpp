@@ -252,12 +252,11 @@ static Bound bound_of(const float* r, float Xc, float Yc, float hx, float hy) {
     ap[i] = fmaf(a[i], hx, b[i] * hy);
     am[i] = fmaf(a[i], hx, -(b[i] * hy));
   }
-  const float n1 = sqrtf(dot3f(pp, pp));
-  const float n2 = sqrtf(dot3f(pm, pm));
+  const float nmax = sqrtf(fmaxf(dot3f(pp, pp), dot3f(pm, pm)));   // = max of the two roots
   const float den = fmaf(-fabsf(ub), hy, fmaf(-fabsf(ua), hx, uu));
   Bound o;
   o.unbounded = !(den > 0.0f);
-  const float tanm = fmaxf(n1, n2) / den;
+  const float tanm = nmax / den;
   const float qn = sqrtf(qq);
   const float dth = qn * tanm;
   o.sd_lo = fmaxf(sdc - dth, 0.0f);

@@ -144,3 +144,19 @@ def test_silhouette_and_adam_entry_points_validate(fea):
     assert lib.fe_adam_step(None, 10, 8, 8, 8, 8, 1e-2, 0.9, 0.999, 1e-8, 0) == 1     # step counts from 1
     assert lib.fe_adam_step(None, 10, 8, 8, 8, 8, 1e-2, 1.0, 0.999, 1e-8, 1) == 1     # beta1 = 1
     assert lib.fe_adam_step(None, 0, None, None, None, None, 1e-2, 0.9, 0.999, 1e-8, 1) == 0  # empty: no-op
+
+
+def test_profile_selection_is_host_only_and_empty_when_nothing_ran(fea):
+    """fe_profile_enable accepts 0, 1 and FE_PROFILE_SELECT | bits; with no launch since the
+    reset every kernel reads zero time and zero launches."""
+    from fuzzy_ellipses_amd import _lib
+    hdr = open(os.path.join(ROOT, "include", "fuzzy_ellipses_amd.h")).read()
+    assert f"#define FE_PROFILE_SELECT 0x{_lib.FE_PROFILE_SELECT:08x}" in hdr
+    _lib.profile_read(reset=True)
+    _lib.profile_enable(True, kernels=["bwd_blend", "blend1"])
+    _lib.profile_enable(True)
+    _lib.profile_enable(False)
+    kt = _lib.profile_read(reset=True)
+    assert set(kt) == set(_lib.KERNELS) and all(v == (0.0, 0) for v in kt.values())
+    with pytest.raises(ValueError):
+        _lib.profile_enable(True, kernels=["no_such_kernel"])
