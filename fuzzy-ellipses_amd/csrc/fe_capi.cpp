@@ -183,6 +183,11 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     set_msg("fe_plan: batch too large (tile-pair capacity above 2^34)");
     return FE_ERR_INVALID_ARGUMENT;
   }
+  if (tiles >= ((int64_t)1 << 31) || stiles >= ((int64_t)1 << 31)) {
+    // one block per tile, and the tile dispatch order holds int32 tile indices
+    set_msg("fe_plan: batch too large (2^31 tiles or more)");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
   L->max_n = max_n;
   L->total_rec = rec;
   L->total_tiles = tiles;

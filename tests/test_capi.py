@@ -160,3 +160,22 @@ def test_profile_selection_is_host_only_and_empty_when_nothing_ran(fea):
     assert set(kt) == set(_lib.KERNELS) and all(v == (0.0, 0) for v in kt.values())
     with pytest.raises(ValueError):
         _lib.profile_enable(True, kernels=["no_such_kernel"])
+
+
+def test_plan_rejects_2_31_tiles(fea):
+    """One block per tile and int32 tile indices in the dispatch order: the planner refuses a
+    batch of 2^31 tiles (here 128 ray-list views of 2^30 rays at tile 8, scenes of n = 0)."""
+    from fuzzy_ellipses_amd import _lib
+    sc = (_lib.FeScene * 1)(_lib.FeScene(0, None, None, None))
+    cam = _lib.FeCamera()
+    cam.fx = cam.fy = 1.0
+
+    def plan(nv):
+        vw = (_lib.FeView * nv)(*[_lib.FeView(0, cam, 1.0, 1.0, None, None, None, 8, 1 << 30) for _ in range(nv)])
+        opt = _lib.FeOptions(8, 64, 0, 0, 0)
+        nbytes = ctypes.c_size_t(0)
+        return _lib.load().fe_plan(sc, 1, vw, nv, ctypes.byref(opt), ctypes.byref(nbytes))
+
+    assert plan(127) == 0
+    assert plan(128) != 0
+    assert b"2^31 tiles" in _lib.load().fe_last_error_string()
