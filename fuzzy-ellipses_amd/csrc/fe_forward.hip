@@ -1241,19 +1241,19 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
                             int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
                             uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
                             uint32_t* tile_np, const int32_t* job_flags, Counters* cnt) {
-  // the group factor is 4 for 8-pixel tiles, 2 for 16, 1 for 32 and for ray lists
+  // the group factor is 4 for 8-pixel tiles, 3 for 16, 1 for 32 and for ray lists
   const dim3 g((unsigned)total_stiles), b(256);
   if (phase == 1) {
     if (ts == 8) hipLaunchKernelGGL((k_tile_mask<1, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
                                     tile_seed, tile_ubm, tile_np, job_flags, cnt);
-    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<1, 2>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
+    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<1, 3>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
                                           tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt);
     else hipLaunchKernelGGL((k_tile_mask<1, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
                             tile_seed, tile_ubm, tile_np, job_flags, cnt);
   } else {
     if (ts == 8) hipLaunchKernelGGL((k_tile_mask<2, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
                                     tile_seed, tile_ubm, tile_np, job_flags, cnt);
-    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 2>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
+    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 3>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
                                           tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt);
     else hipLaunchKernelGGL((k_tile_mask<2, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
                             tile_seed, tile_ubm, tile_np, job_flags, cnt);

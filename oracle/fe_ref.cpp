@@ -293,7 +293,7 @@ static float masked_logit_ub(const Bound& o, float w, float b3l, float b2l) {
 // phase 1 whose depth-logit bound reaches tile_lmin[t] - delta2; a tile is completed with every
 // pair not in phase 1 when the largest masked-logit bound of those pairs, or the phase-1 floor
 // seed tile_seed[t] (NULL: none), reaches tile_lmin[t] + floor_thr.  Membership is pre-tested
-// on groups of sf x sf tiles (sf = 4, 2, 1 for ts = 8, 16, 32): a pair the group bound
+// on groups of sf x sf tiles (sf = 4, 3, 1 for ts = 8, 16, 32): a pair the group bound
 // rejects is rejected for every tile of the group.  Writes counts[T] and the concatenated,
 // per-tile ascending Gaussian ids to `list` (capacity n*T); returns the total.
 struct Geo {
@@ -425,7 +425,7 @@ int64_t fe_ref_tiles(int n, const float* rec, int any_degenerate, int H, int W, 
                      float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
                      const float* tile_lmin, const float* tile_seed, uint32_t* counts, uint32_t* list) {
   const int tx = (W + ts - 1) / ts, ty = (H + ts - 1) / ts;
-  const int sf = (ts == 8) ? 4 : (ts == 16 ? 2 : 1);
+  const int sf = (ts == 8) ? 4 : (ts == 16 ? 3 : 1);
   return tiles_impl(
       n, rec, any_degenerate, tx, ty, sf, [&](int a, int b) { return geo_of(a, b, ts, W, H, cx, cy); },
       [&](int a, int b) { return geo_of(a, b, ts * sf, W, H, cx, cy); }, beta2, beta3, c_alpha, delta2, floor_thr,
