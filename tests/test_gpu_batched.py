@@ -99,6 +99,8 @@ def test_tile_and_batch_do_not_change_results(fea, scenes_mod):
             torch.cuda.synchronize()
             res[(tile, batch)] = [(_np(d), _np(a)) for d, a in outs]
     base = res[(16, 64)] if (16, 64) in res else res[(16, 32)]
+    worst = max(float(np.max(np.abs(d - bd) / np.abs(bd))) for v in res.values() for (d, _), (bd, _) in zip(v, base))
+    print(f"tile/batch configs: largest depth difference {worst:.3e} (relative)")
     for k, v in res.items():
         for (d, a), (bd, ba) in zip(v, base):
             assert np.max(np.abs(a - ba)) <= 1e-6, k
