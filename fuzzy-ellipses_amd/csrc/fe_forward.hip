@@ -299,8 +299,11 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
 // ------------------------------------------------------------------------------------------
 constexpr int kWinWords = 64;   // membership words per LDS window
 
+// Phase 2 with 3x3 groups is held to 128 VGPRs (4 waves per SIMD instead of 3; a few
+// registers spill): 0.171 -> 0.152 ms.  Phase 1 at 5 waves (96 VGPRs) spilled into a slower
+// kernel.
 template <int PHASE, int SFMAX>
-__global__ __launch_bounds__(256) void k_tile_mask(const Job* __restrict__ jobs, int n_jobs, int ts,
+__global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_tile_mask(const Job* __restrict__ jobs, int n_jobs, int ts,
                                                    const float* __restrict__ rec,
                                                    unsigned long long* __restrict__ mask1,
                                                    unsigned long long* __restrict__ mask2,
@@ -983,8 +986,11 @@ struct BlendShape {
   static constexpr int NT = TS * TS / PPT;   // threads per block
 };
 
+// The phase-2 blend of 16-pixel tiles is held to 128 VGPRs (4 waves per SIMD instead of 3):
+// its tiles are short and its time is mostly loading and storing the per-pixel state, which
+// more resident waves hide better (0.112 -> 0.101 ms).
 template <int TS, int GB, int PHASE>
-__global__ __launch_bounds__(BlendShape<TS>::NT) void k_blend(BlendArgs A) {
+__global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 1) void k_blend(BlendArgs A) {
   constexpr int PPT = BlendShape<TS>::PPT;
   constexpr int NT = BlendShape<TS>::NT;
   constexpr int SUB = 8 / PPT;   // records per online-max update: 8 pairs in flight per thread
