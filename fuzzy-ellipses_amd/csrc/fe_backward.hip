@@ -458,15 +458,23 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     // The next chunk's records are loaded before this chunk's atomics are issued: stores
     // and atomics count in the same wait counter as loads, and waiting for records loaded
     // after the atomics would wait for the atomics to complete at memory.
-    // A last chunk of at most 32 Gaussians on a full pixel-grid tile runs on half-waves: lanes
-    // l and l + 32 take the same Gaussian over the two halves of the wave's rows.
+    // A last chunk of at most 32 Gaussians on a full pixel-grid tile runs on half-waves (lanes
+    // l and l + 32 take the same Gaussian over the two halves of the wave's rows), one of at
+    // most 16 on quarter-waves (lanes l, l + 16, l + 32, l + 48 over its four quarters) where a
+    // quarter is whole rows.
     const bool can_halve = rows && npx == TS * TS;
-    const int half_rows = (q / 2) / wv;
-    auto halved = [&](int64_t c0) { return can_halve && total - c0 <= 32; };
+    const bool can_quarter = can_halve && (q / 4) >= wv && (q / 4) % wv == 0;
+    const int half_rows = (q / 2) / wv, quarter_rows = (q / 4) / wv;
+    // lanes per Gaussian in the chunk starting at c0 (block-uniform)
+    auto split_of = [&](int64_t c0) {
+      const int64_t left = total - c0;
+      return (can_quarter && left <= 16) ? 4 : ((can_halve && left <= 32) ? 2 : 1);
+    };
+    auto lane_of = [&](int sp) { return (sp == 4) ? (lane & 15) : ((sp == 2) ? (lane & 31) : lane); };
     int64_t gid_n = 0;
     float4 qa = make_float4(0.f, 0.f, 0.f, 0.f), qb = qa, qc = qa;
     auto fetch = [&](int64_t c0) {
-      const int64_t kk = c0 + (halved(c0) ? (lane & 31) : lane);
+      const int64_t kk = c0 + lane_of(split_of(c0));
       if (kk < total) {
         gid_n = A.dense ? kk : (int64_t)A.list[kk < n1 ? s1 + kk : s2 + (kk - n1)];
         qa = rec24[gid_n * 4 + 0];
@@ -478,8 +486,8 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     asm volatile("" ::"v"(qa.x), "v"(qa.y), "v"(qa.z), "v"(qa.w), "v"(qb.x), "v"(qb.y), "v"(qb.z), "v"(qb.w),
                  "v"(qc.x), "v"(qc.y), "v"(qc.z), "v"(qc.w));
     for (int64_t c0 = 0; c0 < total; c0 += 64) {
-      const bool hv = halved(c0);   // block-uniform
-      const bool act = c0 + (hv ? (lane & 31) : lane) < total;
+      const int sp = split_of(c0);   // block-uniform
+      const bool act = c0 + lane_of(sp) < total;
       const int64_t gid = gid_n;
       const float4 q0 = qa, q1 = qb, q2 = qc;
       float G[11];
@@ -487,9 +495,9 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
       for (int k = 0; k < 11; ++k) G[k] = 0.0f;
       if (act) {
         const float k9[9] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w, q2.x};
-        if (rows && hv) {
+        if (rows && sp > 1) {
           const int px0 = tx * TS, py0 = ty * TS;
-          const int pe = p0 + q / 2, ro = (lane >= 32) ? half_rows : 0;
+          const int pe = p0 + q / sp, ro = (sp == 4) ? (lane >> 4) * quarter_rows : (lane >> 5) * half_rows;
           if (masked && floor_tile)
             bwd_rows_rec2<true, true, true>(pxa, pxb, pxc, p0, pe, ro, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
                                             q2.z, q2.w, b3, b2, kap, G);
@@ -517,8 +525,12 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
           else bwd_pixels_rec2<false, false>(pxa, pxb, pxc, p0, p1, k9, q2.y, q2.z, q2.w, b3, b2, kap, G);
         }
       }
-      if (hv) {
-        // add the other half-wave's sums (same Gaussian in lanes l and l + 32)
+      if (sp > 1) {
+        // add the other half- or quarter-waves' sums (the same Gaussian in lanes l ^ 16, l ^ 32)
+        if (sp == 4) {
+#pragma unroll
+          for (int k = 0; k < 11; ++k) G[k] += __shfl_xor(G[k], 16, 64);
+        }
 #pragma unroll
         for (int k = 0; k < 11; ++k) G[k] += __shfl_xor(G[k], 32, 64);
       }
@@ -539,7 +551,7 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
 #pragma unroll
       for (int i = 0; i < 3; ++i) {
         const int k = wave + 4 * i;
-        if (act && !(hv && lane >= 32) && k < 11) atomicAdd(gsum + (int64_t)k * A.total_rec + gid, v[i]);
+        if (act && lane < 64 / sp && k < 11) atomicAdd(gsum + (int64_t)k * A.total_rec + gid, v[i]);
       }
       __syncthreads();
     }
