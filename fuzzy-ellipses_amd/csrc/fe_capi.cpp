@@ -364,12 +364,11 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   float* lmin = at<float>(ws, L.o_lmin);
   float* seed = lmin + L.total_tiles;   // per-tile floor seed, after the thresholds
   float* ubm = at<float>(ws, L.o_ubm);
-  FE_HIP(launch_init(s, cnt, flags, L.n_jobs));
   bool any_pose = false;
   for (int v = 0; v < n_views; ++v) any_pose = any_pose || views[v].pose_device != nullptr;
   if (any_pose) FE_HIP(launch_pose(s, at<Job>(ws, L.o_jobs), L.n_jobs));
   FE_HIP(timed(s, FE_K_PREPROCESS, [&] {
-    const hipError_t e = launch_order(s, dj, L.n_jobs, L.max_n);
+    const hipError_t e = launch_order(s, dj, L.n_jobs, L.max_n, cnt, flags);   // clears cnt, flags
     return e != hipSuccess ? e : launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, rec2, flags);
   }));
   if (L.has_rays) FE_HIP(launch_ray_rects(s, dj, L.n_jobs, L.total_tiles, L.ts, flags));

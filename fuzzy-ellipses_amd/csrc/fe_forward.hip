@@ -102,8 +102,17 @@ __device__ __forceinline__ uint32_t bitonic_keep(uint32_t v, uint32_t pv, int e,
 
 constexpr int kOrderThreads = kOrderChunk / 4;
 
-__global__ __launch_bounds__(kOrderThreads) void k_order(const Job* __restrict__ jobs) {
+// It also clears the forward's counters and job flags (block (0, 0) the counters, block
+// (0, j) job j's flag), which k_init does when no job has Gaussians: one launch fewer.
+__global__ __launch_bounds__(kOrderThreads) void k_order(const Job* __restrict__ jobs, Counters* __restrict__ cnt,
+                                                         int32_t* __restrict__ job_flags) {
   __shared__ uint32_t key[kOrderChunk];
+  if (blockIdx.x == 0) {
+    if (threadIdx.x == 0) job_flags[blockIdx.y] = 0;
+    if (blockIdx.y == 0)
+      for (int k = threadIdx.x; k < (int)(sizeof(Counters) / 8); k += kOrderThreads)
+        reinterpret_cast<unsigned long long*>(cnt)[k] = 0ull;
+  }
   const Job& jb = jobs[blockIdx.y];
   const int c0 = blockIdx.x * kOrderChunk;
   if (c0 >= jb.n || !jb.order) return;   // block-uniform
@@ -1228,10 +1237,10 @@ hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t 
   return hipGetLastError();
 }
 
-hipError_t launch_order(hipStream_t s, const Job* jobs, int n_jobs, int max_n) {
-  if (max_n <= 0) return hipSuccess;
+hipError_t launch_order(hipStream_t s, const Job* jobs, int n_jobs, int max_n, Counters* cnt, int32_t* job_flags) {
+  if (max_n <= 0) return launch_init(s, cnt, job_flags, n_jobs);
   hipLaunchKernelGGL(k_order, dim3((max_n + kOrderChunk - 1) / kOrderChunk, n_jobs), dim3(kOrderThreads), 0, s,
-                     jobs);
+                     jobs, cnt, job_flags);
   return hipGetLastError();
 }
 

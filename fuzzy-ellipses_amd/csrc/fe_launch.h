@@ -18,7 +18,8 @@ hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_j
 hipError_t launch_pose(hipStream_t s, Job* jobs, int n_jobs);
 hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t total_tiles, int ts,
                             int32_t* job_flags);
-hipError_t launch_order(hipStream_t s, const Job* jobs, int n_jobs, int max_n);
+// Also clears the forward's counters and job flags (as launch_init).
+hipError_t launch_order(hipStream_t s, const Job* jobs, int n_jobs, int max_n, Counters* cnt, int32_t* job_flags);
 hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec, float* rec2,
                              int32_t* job_flags);
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,

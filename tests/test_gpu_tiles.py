@@ -75,3 +75,13 @@ def test_dispatch_order_all_shell_views(fea, scenes_mod):
     _check_order(st)
     k = np.diff(st["off1"])[st["tile_order"][0]]
     print(f"16 views: {len(k)} tiles, phase-1 lengths in dispatch order {k[:3]} ... {k[-3:]}")
+
+
+def test_keys_and_order_on_81920_tiles(fea, scenes_mod):
+    """81920 tiles (a 2048x2560 view at tile 8; the scan's blocks span several buckets and
+    views' worth of tiles): lists, ranges and both dispatch orders as for small batches."""
+    scene = scenes_mod.single_view_scene(seed=5, n=64, H=2048, W=2560, f=2000.0)
+    cam = scene.cameras[0]
+    assert (cam.H // 8) * (cam.W // 8) > 65536
+    k1, k2 = _check(fea, scene, cam, 8)
+    print(f"2048x2560 tile 8: phase-1 pairs {k1}, phase-2 pairs {k2}")
