@@ -291,8 +291,8 @@ def main():
         print(json.dumps({"copy_calibration_bytes_read": src.numel() * 4, "bytes_written": dst.numel() * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              f"(views and scenes shard across GPUs with no exchange; see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (views and scenes "
+              f"shard across GPUs with no exchange, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import numpy as np
