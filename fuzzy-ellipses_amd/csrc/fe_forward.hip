@@ -975,7 +975,8 @@ struct BlendArgs {
   const unsigned long long* off1;
   const unsigned long long* off2;
   const uint32_t* list;
-  float* st;            // per-pixel state, 6 planes of total_pix: M S Z E Zall D
+  float* st;            // per-pixel state, 6 planes of total_pix: M S Z E Zall D (Zall and the
+                        // NaN flag in the sixth plane feed the second pass; the backward reads M S Z E D)
   int64_t total_pix;
   float* tile_lmin;
   float* tile_seed;     // phase 1 writes the floor seed of each tile (phase-2 membership input)
@@ -1176,11 +1177,13 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
       jb.depth[pix[q] - jb.pix_base] = D;
       jb.alpha[pix[q] - jb.pix_base] = alpha;
       if (A.save) {
-        A.st[pix[q]] = M[q];
-        A.st[A.total_pix + pix[q]] = S[q];
-        A.st[2 * A.total_pix + pix[q]] = Z[q];
-        A.st[3 * A.total_pix + pix[q]] = E[q];
-        A.st[4 * A.total_pix + pix[q]] = Zall[q];
+        // what the backward reads: M, S, Z, E (unchanged by a second pass with no pairs) and D
+        if (PHASE == 0 || end > start) {
+          A.st[pix[q]] = M[q];
+          A.st[A.total_pix + pix[q]] = S[q];
+          A.st[2 * A.total_pix + pix[q]] = Z[q];
+          A.st[3 * A.total_pix + pix[q]] = E[q];
+        }
         A.st[5 * A.total_pix + pix[q]] = D;
       }
     }
