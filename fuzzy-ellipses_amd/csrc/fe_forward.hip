@@ -1054,7 +1054,8 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
       E[q] = A.st[3 * A.total_pix + pix[q]];
       Zall[q] = A.st[4 * A.total_pix + pix[q]];
       // a NaN logit seen by the phase-1 pass (kept in the sixth state plane between the passes)
-      nanf[q] = A.st[5 * A.total_pix + pix[q]] != 0.0f;
+      // (only the NaN-safe blend of a view with degenerate records sets it)
+      nanf[q] = safe && A.st[5 * A.total_pix + pix[q]] != 0.0f;
     } else {
       M[q] = -3.402823466e38f; S[q] = 0.f; Z[q] = 0.f; E[q] = 0.f; Zall[q] = 0.f;
       nanf[q] = false;
@@ -1139,7 +1140,7 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
         A.st[2 * A.total_pix + pix[q]] = Z[q];
         A.st[3 * A.total_pix + pix[q]] = E[q];
         A.st[4 * A.total_pix + pix[q]] = Zall[q];
-        A.st[5 * A.total_pix + pix[q]] = nanf[q] ? 1.0f : 0.0f;
+        if (safe) A.st[5 * A.total_pix + pix[q]] = nanf[q] ? 1.0f : 0.0f;
         const bool pos = S[q] > 0.0f && !nanf[q];
         const float l2s = pos ? __log2f(S[q]) : 0.0f;
         lm = fminf(lm, pos ? M[q] + l2s : -INFINITY);
