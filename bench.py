@@ -25,6 +25,8 @@ METRIC = "ray-Gaussian evals/s + fwd+bwd frames/s @480x640,N=4096; served p50/p9
 FP32_PEAK_TFLOPS = 157.3          # MI355X published dense FP32 vector peak
 FLOPS_FWD = 36                    # per evaluated (pixel, Gaussian) pair, blend forward
 FLOPS_BWD = 93                    # per evaluated pair, blend backward incl. recompute
+TRANS_PER_PAIR = 3                # v_exp x2 + v_rcp per evaluated pair, forward and backward
+TRANS_PEAK_TOPS = 9.8             # 256 CUs x 4 SIMDs x 16 lanes / 4 (quarter rate) x 2.4 GHz
 
 
 
@@ -386,8 +388,12 @@ def main():
     kavg = {k: (ms / n if n else 0.0) for k, (ms, n) in kt_all.items()}
     avg_s = (kt[dom][0] / kt[dom][1] if kt[dom][1] else 0.0) / 1e3
     achieved = flops_per_launch[dom] / avg_s / 1e12 if avg_s > 0 else 0.0
+    pairs_dom = flops_per_launch[dom] / (FLOPS_BWD if dom == "bwd_blend" else FLOPS_FWD)
+    trans = TRANS_PER_PAIR * pairs_dom / avg_s / 1e12 if avg_s > 0 else 0.0
     roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
             "launch_avg_ms": round(avg_s * 1e3, 5),
+            "trans_achieved": round(trans, 3), "trans_peak": TRANS_PEAK_TOPS, "trans_unit": "Top/s",
+            "trans_frac": round(trans / TRANS_PEAK_TOPS, 4),
             "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, args), "kernel": dom,
             "note": "compute-bound on FP32 VALU (no MFMA, no packed FP32); flops/launch = "
                     f"{FLOPS_FWD if dom != 'bwd_blend' else FLOPS_BWD} x evaluated pairs"}
