@@ -7,8 +7,8 @@ This mirrors the reference's caller-facing interface for the render path:
   alpha images computed by the HIP kernels;
 * ``rasterize``: the argument list of the reference binding
   ``fuzzy_ellipses/__init__.py:189-236`` (scales/quaternions, camera rays, intrinsics),
-  converting scales/quaternions to the precision factor on the host side and applying the
-  binding's ``nan_to_num`` to the outputs;
+  through the reference-shaped custom-call target ``rasterize_gaussians_fwd`` (``xla.py``),
+  with the binding's ``nan_to_num`` applied to the outputs;
 * ``Renderer``: batched forward/backward over many scenes and views in one launch per
   kernel, with a reusable device workspace.
 
@@ -144,9 +144,23 @@ class Renderer:
                          _lib.FE_FLAG_REUSE_DESCRIPTORS if reuse else 0, self.alpha_eps, self.depth_eps)
 
     # -- planning -----------------------------------------------------------------------
+    @staticmethod
+    def _view_key(view):
+        """Every value the C structs copy from a view: a cached struct is reused only when all
+        of them are equal (object ids are not used: CPython reuses them after a free)."""
+        si, cam, b2, b3 = view
+        if isinstance(cam, RayList):
+            ck = ("rays", cam.R.astype(np.float32).tobytes(), cam.t.astype(np.float32).tobytes(),
+                  cam.xy.data_ptr(), cam.n)
+        else:
+            ck = (np.asarray(cam.R, dtype=np.float32).tobytes(), np.asarray(cam.t, dtype=np.float32).tobytes(),
+                  int(cam.H), int(cam.W), float(np.float32(cam.fx)), float(np.float32(cam.fy)),
+                  float(np.float32(cam.cx)), float(np.float32(cam.cy)))
+        return (int(si), ck, float(np.float32(b2)), float(np.float32(b3)))
+
     def _structs(self, scenes: Sequence[SceneTensors], views, outs):
-        key = (tuple(id(s) for s in scenes), tuple((id(v[1]), v[0], v[2], v[3]) for v in views),
-               tuple((_ptr(d), _ptr(a)) for d, a in outs))
+        key = (tuple((s.n, _ptr(s.means), _ptr(s.prec), _ptr(s.weights_log)) for s in scenes),
+               tuple(self._view_key(v) for v in views), tuple((_ptr(d), _ptr(a)) for d, a in outs))
         if self._cache is not None and self._cache[0] == key:
             return self._cache[1], self._cache[2]
         sc = (FeScene * len(scenes))(*[s.fe() for s in scenes])
@@ -156,6 +170,14 @@ class Renderer:
             vw[i] = _fe_view(si, cam, b2, b3, _ptr(d), _ptr(a))
         self._cache = (key, sc, vw)
         return sc, vw
+
+    def _check_image(self, t, cam, what):
+        """A per-view image buffer: contiguous float32 on the render device with one value per
+        pixel (H*W) or ray (n); the kernels index it by pixel."""
+        n = cam.n if isinstance(cam, RayList) else int(cam.H) * int(cam.W)
+        if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_contiguous() \
+                or t.device != self.device or t.numel() != n:
+            raise FeError(f"{what}: expected a contiguous float32 tensor of {n} values on {self.device}")
 
     def workspace_bytes(self, scenes, views, save_for_backward=False) -> int:
         sc = (FeScene * len(scenes))(*[s.fe() for s in scenes])
@@ -172,7 +194,6 @@ class Renderer:
         if self._ws is None or self._ws.numel() < nbytes:
             self._ws = None
             self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
-            self._cache = None
         return self._ws
 
     def alloc_outputs(self, views):
@@ -193,6 +214,12 @@ class Renderer:
         """
         if outs is None:
             outs = self.alloc_outputs(views)
+        elif len(outs) != len(views):
+            raise FeError("forward: outs needs one (depth, alpha) pair per view")
+        else:
+            for (d, a), (_, cam, _, _) in zip(outs, views):
+                self._check_image(d, cam, "forward: depth output")
+                self._check_image(a, cam, "forward: alpha output")
         sc, vw = self._structs(scenes, views, outs)
         opt = self._opt(save_for_backward)
         nbytes = ctypes.c_size_t(0)
@@ -201,11 +228,10 @@ class Renderer:
         if reuse_descriptors and (self._ws is None or self._ws.numel() < nbytes.value):
             raise FeError("forward: reuse_descriptors needs a prior call with the same arguments")
         ws = self._workspace(int(nbytes.value))
-        sc, vw = self._structs(scenes, views, outs)
         opt = self._opt(save_for_backward, reuse_descriptors)
         _lib.check(self.lib.fe_forward(_stream(self.device), sc, len(scenes), vw, len(views), ctypes.byref(opt),
                                        ws.data_ptr(), ws.numel()), "fe_forward")
-        self._last = (list(scenes), sc, vw, outs) if save_for_backward else None
+        self._last = (list(scenes), sc, vw, outs, list(views)) if save_for_backward else None
         return outs
 
     def stats(self) -> dict:
@@ -247,17 +273,30 @@ class Renderer:
         to accumulate into; by default fresh zeroed tensors are returned."""
         if self._last is None:
             raise FeError("backward: the last forward was not run with save_for_backward=True")
-        scenes, sc, vw, outs = self._last
+        scenes, sc, vw, outs, views = self._last
+        if len(d_depth) != len(outs) or len(d_alpha) != len(outs):
+            raise FeError("backward: need one depth and one alpha cotangent (or None) per view")
         cot = (FeViewCotangent * len(outs))()
         for i in range(len(outs)):
             dd, da = d_depth[i], d_alpha[i]
-            for t in (dd, da):
-                if t is not None and (t.dtype != torch.float32 or not t.is_contiguous() or t.device != self.device):
-                    raise FeError("backward: cotangents must be contiguous float32 tensors on the render device")
+            cam = views[i][1]
+            for t, what in ((dd, "depth"), (da, "alpha")):
+                if t is not None:
+                    self._check_image(t, cam, f"backward: {what} cotangent of view {i}")
             cot[i] = FeViewCotangent(_ptr(dd), _ptr(da))
         if grads is None:
             grads = [(torch.zeros_like(s.means), torch.zeros_like(s.prec), torch.zeros_like(s.weights_log))
                      for s in scenes]
+        if len(grads) != len(scenes):
+            raise FeError("backward: need one (d_means, d_prec, d_weights_log) triple per scene")
+        for s, g in zip(scenes, grads):
+            if len(g) != 3:
+                raise FeError("backward: need one (d_means, d_prec, d_weights_log) triple per scene")
+            for t, ref in zip(g, (s.means, s.prec, s.weights_log)):
+                if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_contiguous() \
+                        or t.device != self.device or t.shape != ref.shape:
+                    raise FeError(f"backward: gradient buffers must be contiguous float32 tensors on "
+                                  f"{self.device} shaped like the scene's (means, prec, weights_log)")
         gs = (FeSceneGrad * len(scenes))(*[FeSceneGrad(_ptr(a), _ptr(b), _ptr(c)) for a, b, c in grads])
         opt = self._opt(True, reuse_descriptors)
         ws = self._ws
@@ -327,30 +366,58 @@ def prec_from_scale_quat(scales, quats) -> np.ndarray:
     return np.triu(np.transpose(np.linalg.inv(C), (0, 2, 1)))
 
 
+def projection_matrix(width, height, fx, fy, cx, cy, znear, zfar) -> np.ndarray:
+    """The reference binding's projection matrix (fuzzy_ellipses/__init__.py:36-76, transposed
+    as it is there; cx and cy do not enter).  It is passed to the custom call, which accepts
+    and ignores it like the view matrix."""
+    tan_x = math.tan(math.atan(width / 2 / fx))
+    tan_y = math.tan(math.atan(height / 2 / fy))
+    top, right = tan_y * znear, tan_x * znear
+    bottom, left = -top, -right
+    P = np.array([[2.0 * znear / (right - left), 0.0, (right + left) / (right - left), 0.0],
+                  [0.0, 2.0 * znear / (top - bottom), (top + bottom) / (top - bottom), 0.0],
+                  [0.0, 0.0, zfar / (zfar - znear), -(zfar * znear) / (zfar - znear)],
+                  [0.0, 0.0, 1.0, 0.0]])
+    return P.T
+
+
 def rasterize(means3D, scales, rotations, weights, camera_rays, camera_rot, camera_trans, image_height,
               image_width, fx, fy, cx, cy, near=0.01, far=100.0, device="cuda"):
-    """Argument list of the reference binding (fuzzy_ellipses/__init__.py:189-236).
+    """Argument list and results of the reference binding (fuzzy_ellipses/__init__.py:189-236).
 
-    ``camera_rays`` must be the pinhole grid ((x-cx)/fx, (y-cy)/fy, 1) of the given
-    intrinsics (it is spot-checked); ``near``/``far`` only fed the reference's culling
-    projection and are unused.  Uses the reference's fixed beta2=21.4, beta3=2.66
-    (rasterizer_impl.cu:196-197) and returns nan_to_num'd (depth, alpha) of shape (H, W).
+    Runs the reference's custom-call target ``rasterize_gaussians_fwd`` of libfe_amd.so with
+    the reference's 11 buffers and 20-byte descriptor (``xla.call_custom_target``): scales and
+    quaternions become the precision factor on the device, ``camera_rays`` (H*W, 3) may be any
+    camera-frame rays with third component 1 (rows with another third component get a NaN
+    depth, zeroed here), beta2=21.4 and beta3=2.66 as the reference fixes them
+    (rasterizer_impl.cu:196-197).  Returns the binding's nan_to_num'd (depth, alpha) of shape
+    (H, W) as float32 device tensors.
     """
+    from . import xla
     H, W = int(image_height), int(image_width)
-    rays = np.asarray(camera_rays, dtype=np.float64).reshape(-1, 3)
-    if rays.shape[0] != H * W:
-        raise ValueError("rasterize: camera_rays must have H*W rows")
-    idx = np.array([0, W - 1, (H - 1) * W, H * W - 1, (H // 2) * W + W // 2])
-    xs, ys = (idx % W).astype(np.float64), (idx // W).astype(np.float64)
-    expect = np.stack([(xs - cx) / fx, (ys - cy) / fy, np.ones_like(xs)], 1)
-    if not np.allclose(rays[idx], expect, rtol=1e-5, atol=1e-6):
-        raise ValueError("rasterize: camera_rays is not the pinhole grid of (fx, fy, cx, cy)")
-    prec = prec_from_scale_quat(scales, rotations)
-    cam = Camera(R=np.asarray(camera_rot, dtype=np.float64), t=np.asarray(camera_trans, dtype=np.float64),
-                 H=H, W=W, fx=float(fx), fy=float(fy), cx=float(cx), cy=float(cy))
-    d, a = render_camera(np.asarray(means3D), prec, np.asarray(weights), cam, BETA2, BETA3, device=device)
-    d = torch.nan_to_num(d, nan=0.0, posinf=10.0, neginf=0.0)
-    a = torch.nan_to_num(a, nan=0.0, posinf=1.0, neginf=0.0)
+    dev = torch.device(device)
+    if dev.type == "cuda" and dev.index is None:
+        dev = torch.device("cuda", torch.cuda.current_device())
+    f32 = lambda a: torch.as_tensor(np.asarray(a) if not isinstance(a, torch.Tensor) else a,
+                                    dtype=torch.float32, device=dev).contiguous()
+    means, w, sc, q = f32(means3D), f32(weights), f32(scales), f32(rotations)
+    rays, R, t = f32(camera_rays), f32(camera_rot), f32(camera_trans)
+    n = int(w.shape[0])
+    if means.shape != (n, 3) or sc.shape != (n, 3) or q.shape != (n, 4):
+        raise ValueError("rasterize: expected means3D (N,3), scales (N,3), rotations (N,4), weights (N,)")
+    if rays.numel() != 3 * H * W or R.numel() != 9 or t.numel() != 3:
+        raise ValueError("rasterize: expected camera_rays (H*W,3), camera_rot (3,3), camera_trans (3,)")
+    view = f32(np.eye(4))
+    proj = f32(projection_matrix(W, H, fx, fy, cx, cy, near, far))
+    tan_fovx = math.tan(math.atan(W / 2 / fx) * 2.0)
+    tan_fovy = math.tan(math.atan(H / 2 / fy) * 2.0)
+    depth = torch.empty((H, W), dtype=torch.float32, device=dev)
+    alpha = torch.empty((H, W), dtype=torch.float32, device=dev)
+    opaque = xla.build_gaussian_rasterize_descriptor(H, W, n, tan_fovx, tan_fovy)
+    bufs = [means, w, sc, q, rays, R, t, view, proj, depth, alpha]
+    xla.call_custom_target(_stream(dev), [b.data_ptr() for b in bufs], opaque)
+    d = torch.nan_to_num(depth, nan=0.0, posinf=10.0, neginf=0.0)
+    a = torch.nan_to_num(alpha, nan=0.0, posinf=1.0, neginf=0.0)
     return d, a
 
 

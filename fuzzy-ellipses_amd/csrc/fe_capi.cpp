@@ -27,6 +27,7 @@ static void set_err(const char* what, hipError_t e) {
   snprintf(g_err, sizeof(g_err), "%s: %s", what, hipGetErrorString(e));
 }
 static void set_msg(const char* what) { snprintf(g_err, sizeof(g_err), "%s", what); }
+void set_error_message(const char* msg) { set_msg(msg); }
 
 #define FE_HIP(call)                         \
   do {                                       \
@@ -63,7 +64,7 @@ struct WsHeader {
 constexpr uint64_t kMagic = 0x4645414d44313031ull;  // "FEAMD101"
 
 static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* views, int32_t n_views,
-                      const fe_options* opt, Layout* L, std::vector<Job>* jobs) {
+                      const fe_options* opt, Layout* L, Job* jobs) {
   if (!scenes || !views || n_scenes <= 0 || n_views <= 0) {
     set_msg("fe_plan: need at least one scene and one view");
     return FE_ERR_INVALID_ARGUMENT;
@@ -96,7 +97,6 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->save = o.save_for_backward ? 1 : 0;
   L->reuse = (o.flags & FE_FLAG_REUSE_DESCRIPTORS) ? 1 : 0;
   L->n_jobs = n_views;
-  if (jobs) jobs->resize(n_views);
   int64_t rec = 0, tiles = 0, stiles = 0, maskw = 0, pix = 0, cap = 0;
   // membership pre-test groups: 32x32 pixels for 8-pixel tiles, 48x48 for 16 (measured against
   // 32x32: membership 18% (phase 1) and 13% (phase 2) faster; 64x64 slows phase 2, whose 16
@@ -136,7 +136,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     if (ray_view) L->has_rays = 1;
     const int nw = (sc.n + 63) / 64;
     if (jobs) {
-      Job& jb = (*jobs)[v];
+      Job& jb = jobs[v];
       memset(&jb, 0, sizeof(Job));
       jb.means = sc.means;
       jb.prec = sc.prec;
@@ -293,8 +293,8 @@ static Profiler g_prof;
 
 // Runs `launch` between two recorded events when kernel k is being timed.
 template <typename F>
-static hipError_t timed(hipStream_t s, int k, F launch) {
-  if (!((g_prof.mask >> k) & 1u)) return launch();
+static hipError_t timed(hipStream_t s, int k, F launch, bool may_time = true) {
+  if (!may_time || !((g_prof.mask >> k) & 1u)) return launch();
   hipEvent_t a, b;
   {
     std::lock_guard<std::mutex> lk(g_prof.mu);
@@ -312,16 +312,28 @@ static hipError_t timed(hipStream_t s, int k, F launch) {
   return e;
 }
 
+// Descriptors of a call that allocates nothing, passed by value as a kernel argument and
+// copied into the workspace (one view's header and job fit in 1 KB).
+struct ArgBlob {
+  unsigned char b[1024];
+};
+__global__ void k_put_blob(ArgBlob blob, int bytes, unsigned char* dst) {
+  for (int i = threadIdx.x; i < bytes; i += blockDim.x) dst[i] = blob.b[i];
+}
+
 template <typename T>
 static T* at(void* ws, size_t off) {
   return reinterpret_cast<T*>(reinterpret_cast<char*>(ws) + off);
 }
 
-fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
-                  int32_t n_views, const fe_options* opt, void* ws, size_t ws_bytes) {
+// The forward of one call.  `jobs` has room for n_views descriptors.  With `noalloc` the call
+// allocates nothing: the descriptors travel to the workspace as a kernel argument instead of
+// through the pinned upload ring, and no launch is event-timed.
+static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
+                              int32_t n_views, const fe_options* opt, void* ws, size_t ws_bytes, Job* jobs,
+                              bool noalloc) {
   Layout L;
-  std::vector<Job> jobs;
-  fe_status st = plan(scenes, n_scenes, views, n_views, opt, &L, &jobs);
+  fe_status st = plan(scenes, n_scenes, views, n_views, opt, &L, jobs);
   if (st != FE_OK) return st;
   if (!ws || ws_bytes < L.bytes) {
     set_msg("fe_forward: workspace missing or too small (see fe_plan)");
@@ -344,16 +356,29 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
     jobs[v].rank = at<int32_t>(ws, L.o_rank) + jobs[v].rec_base;
   }
   // header + jobs in one upload
-  std::vector<char> blob(L.o_counters);
-  WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode,
-             L.o_rec, L.o_list, L.o_lmin, (uint64_t)L.total_rec, L.o_state, (uint64_t)L.total_pix,
-             L.o_order, L.o_tord};
-  memcpy(blob.data() + L.o_header, &h, sizeof(h));
-  memcpy(blob.data() + L.o_jobs, jobs.data(), sizeof(Job) * n_views);
-  if (!L.reuse) {
+  const WsHeader h{kMagic, L.o_counters, L.o_off1, L.o_off2, (uint64_t)L.total_tiles, (uint64_t)L.mode,
+                   L.o_rec, L.o_list, L.o_lmin, (uint64_t)L.total_rec, L.o_state, (uint64_t)L.total_pix,
+                   L.o_order, L.o_tord};
+  if (!L.reuse && noalloc) {
+    if (L.o_counters > sizeof(ArgBlob)) {
+      set_msg("forward: too many views for descriptors passed as a kernel argument");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+    ArgBlob blob;
+    memset(&blob, 0, sizeof(blob));
+    memcpy(blob.b + L.o_header, &h, sizeof(h));
+    memcpy(blob.b + L.o_jobs, jobs, sizeof(Job) * n_views);
+    hipLaunchKernelGGL(k_put_blob, dim3(1), dim3(256), 0, s, blob, (int)L.o_counters,
+                       static_cast<unsigned char*>(ws));
+    FE_HIP(hipGetLastError());
+  } else if (!L.reuse) {
+    std::vector<char> blob(L.o_counters);
+    memcpy(blob.data() + L.o_header, &h, sizeof(h));
+    memcpy(blob.data() + L.o_jobs, jobs, sizeof(Job) * n_views);
     st = upload(s, ws, blob.data(), blob.size());
     if (st != FE_OK) return st;
   }
+  const bool may_time = !noalloc;
 
   const Job* dj = at<Job>(ws, L.o_jobs);
   Counters* cnt = at<Counters>(ws, L.o_counters);
@@ -370,13 +395,13 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   FE_HIP(timed(s, FE_K_PREPROCESS, [&] {
     const hipError_t e = launch_order(s, dj, L.n_jobs, L.max_n, cnt, flags);   // clears cnt, flags
     return e != hipSuccess ? e : launch_preprocess(s, dj, L.n_jobs, L.max_n, rec, rec2, flags);
-  }));
+  }, may_time));
   if (L.has_rays) FE_HIP(launch_ray_rects(s, dj, L.n_jobs, L.total_tiles, L.ts, flags));
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
       return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, rec2, nullptr, nullptr, nullptr,
                           nullptr, state, L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
-    }));
+    }, may_time));
     return FE_OK;
   }
   unsigned long long* mask = at<unsigned long long>(ws, L.o_mask);
@@ -393,31 +418,40 @@ fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const
   FE_HIP(timed(s, FE_K_MASK1, [&] {
     return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, seed, ubm, tnp,
                             flags, cnt);
-  }));
+  }, may_time));
   FE_HIP(timed(s, FE_K_SCAN1, [&] {
     return launch_scan(s, count, L.total_tiles, partial, off1, nullptr, nullptr, thist, tord);
-  }));
-  FE_HIP(timed(s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }));
+  }, may_time));
+  FE_HIP(timed(
+      s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }, may_time));
   FE_HIP(timed(s, FE_K_BLEND1, [&] {
     return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
                         L.total_pix, lmin, seed, ubm, cnt, flags, L.save, tord);
-  }));
+  }, may_time));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
     return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, seed, ubm, tnp,
                             flags, cnt);
-  }));
+  }, may_time));
   FE_HIP(timed(s, FE_K_SCAN2, [&] {
     // the backward's order: both lists of a tile (phase-1 length from off1)
     return launch_scan(s, count, L.total_tiles, partial, off2, off1 + L.total_tiles, off1,
                        thist + kTileBuckets, tord + L.total_tiles);
-  }));
-  FE_HIP(timed(s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }));
+  }, may_time));
+  FE_HIP(timed(
+      s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }, may_time));
   FE_HIP(timed(s, FE_K_BLEND2, [&] {
     return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
                         L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
-  }));
+  }, may_time));
   return FE_OK;
+}
+
+fe_status forward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
+                  int32_t n_views, const fe_options* opt, void* ws, size_t ws_bytes) {
+  std::vector<Job> jobs(n_views > 0 ? (size_t)n_views : 0);
+  return forward_impl(s, scenes, n_scenes, views, n_views, opt, ws, ws_bytes, jobs.empty() ? nullptr : jobs.data(),
+                      false);
 }
 
 fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
@@ -427,8 +461,7 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
   if (opt) o = *opt;
   o.save_for_backward = 1;
   Layout L;
-  std::vector<Job> jobs;
-  fe_status st = plan(scenes, n_scenes, views, n_views, &o, &L, &jobs);
+  fe_status st = plan(scenes, n_scenes, views, n_views, &o, &L, nullptr);
   if (st != FE_OK) return st;
   if (!cot || !grads) {
     set_msg("fe_backward: null cotangent or gradient array");

@@ -14,6 +14,9 @@ struct Counters;
 // Blocks per view of the silhouette objective kernels (partial sums per view).
 constexpr int kSilBlocksPerView = 64;
 
+// Sets the text fe_last_error_string reports (this thread).
+void set_error_message(const char* msg);
+
 hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs);
 hipError_t launch_pose(hipStream_t s, Job* jobs, int n_jobs);
 hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t total_tiles, int ts,

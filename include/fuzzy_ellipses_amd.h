@@ -195,6 +195,40 @@ fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float
                         float beta3, float* depth, float* alpha, void* workspace,
                         size_t workspace_bytes);
 
+/* ---- the reference's XLA custom-call target, unchanged in shape ----
+ * Replaces RasterizeGaussiansCUDAJAX (rasterize_main.h:26-30, rasterize_main.cu:29-114),
+ * which ext.cpp:22-32 exports as the capsule "rasterize_gaussians_fwd" and
+ * fuzzy_ellipses/__init__.py:22-23,111-166 registers and lowers.  Same signature, same
+ * 11 device buffers in the same order, same 20-byte opaque descriptor
+ * (rasterize_main.h:13-19, packed by ext.cpp:34-42 / bindings.h:22-37):
+ *   buffers[0] means (N,3)   [1] log-weights (N,)   [2] scales (N,3)   [3] quaternions (N,4),
+ *              scalar first
+ *   [4] camera rays (H*W,3), camera frame, row-major pixels   [5] R (3,3)   [6] t (3,)
+ *   [7] view matrix (4,4)   [8] projection matrix (4,4)   (7, 8 and tan_fov are accepted and
+ *       unused: they fed the reference's culling heuristic, not the math)
+ *   [9] depth (H,W) written   [10] alpha (H,W) written
+ * The precision factor is computed on the device from scales and quaternions (U upper
+ * triangular, U U^T = (R(q) diag(s^2) R(q)^T)^-1, R(q) = quat_to_rot of
+ * baseline/util_render.py:51-76); beta2 = 21.4 and beta3 = 2.66 as the reference fixes them
+ * (rasterizer_impl.cu:196-197).  A ray (x, y, z) is used as given; for z != 1 alpha is exact
+ * and depth is NaN (t*, and so the depth softmax, depends on the ray's length, which the
+ * camera-plane ray list does not carry).  N = 0 writes zeros, as the reference does.
+ * Asynchronous on `stream`; scratch memory is library-owned per (device, stream) and grows
+ * (with one stream synchronisation) only when a call needs more than any before.
+ * rasterize_gaussians_fwd returns nothing, like the reference; on failure it fills both
+ * outputs with NaN and leaves the reason in fe_last_error_string.
+ * fe_rasterize_gaussians_fwd is the same call returning the status. */
+typedef struct fe_rasterize_descriptor {
+  int32_t image_height;
+  int32_t image_width;
+  int32_t P;            /* number of Gaussians */
+  float tan_fovx;
+  float tan_fovy;
+} fe_rasterize_descriptor;
+
+void rasterize_gaussians_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+fe_status fe_rasterize_gaussians_fwd(void* stream, void** buffers, const char* opaque, size_t opaque_len);
+
 /* ---- shape-from-silhouette caller: objective + cotangents, Adam ----
  * The optimisation loop that calls fe_forward / fe_backward each step
  * (shape_from_silhouette.ipynb: "jax_objective" = mean BCE of alpha clipped to

@@ -15,7 +15,7 @@ def _declared():
     for fn in os.listdir(os.path.join(ROOT, "include")):
         if fn.endswith(".h"):
             txt = open(os.path.join(ROOT, "include", fn)).read()
-            names |= set(re.findall(r"^\s*(?:[\w\*]+\s+)+\**(fe_\w+)\s*\(", txt, re.M))
+            names |= set(re.findall(r"^\s*(?:[\w\*]+\s+)+\**(fe_\w+|rasterize_gaussians_fwd)\s*\(", txt, re.M))
     return sorted(names)
 
 
@@ -179,3 +179,38 @@ def test_plan_rejects_2_31_tiles(fea):
     assert plan(127) == 0
     assert plan(128) != 0
     assert b"2^31 tiles" in _lib.load().fe_last_error_string()
+
+
+def test_custom_call_registration_and_descriptor(fea):
+    """The reference binding's `_C` surface (ext.cpp:22-47): a capsule named
+    "xla._CUSTOM_CALL_TARGET" around `rasterize_gaussians_fwd`, and the 20-byte descriptor
+    {int32 H, W, P; float tan_fovx, tan_fovy} (rasterize_main.h:13-19)."""
+    import struct
+    from fuzzy_ellipses_amd import xla
+    regs = xla.registrations()
+    assert set(regs) == {"rasterize_gaussians_fwd"}
+    cap = regs["rasterize_gaussians_fwd"]
+    get_name = ctypes.pythonapi.PyCapsule_GetName
+    get_name.restype = ctypes.c_char_p
+    get_name.argtypes = [ctypes.py_object]
+    assert get_name(cap) == b"xla._CUSTOM_CALL_TARGET"
+    get_ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = ctypes.c_void_p
+    get_ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    assert get_ptr(cap, b"xla._CUSTOM_CALL_TARGET") == xla.target_address() != 0
+    d = xla.build_gaussian_rasterize_descriptor(480, 640, 4096, 1.5, 0.75)
+    assert len(d) == 20
+    assert struct.unpack("<iiiff", d) == (480, 640, 4096, 1.5, 0.75)
+
+
+def test_custom_call_rejects_bad_descriptor_without_gpu(fea):
+    """Argument checks run before any device work: a descriptor of the wrong size or a
+    negative count is refused with a status (the void target then touches no buffer)."""
+    from fuzzy_ellipses_amd import _lib, xla
+    lib = _lib.load()
+    bufs = (ctypes.c_void_p * 11)()
+    op = ctypes.create_string_buffer(b"\0" * 19, 19)
+    assert lib.fe_rasterize_gaussians_fwd(None, bufs, op, 19) == 1
+    bad = xla.build_gaussian_rasterize_descriptor(4, 4, -1, 1.0, 1.0)
+    assert lib.fe_rasterize_gaussians_fwd(None, bufs, bad, 20) == 1
+    assert b"rasterize_gaussians_fwd" in lib.fe_last_error_string()
