@@ -20,7 +20,7 @@ EXPORTS = [
     "fe_backward", "fe_read_stats", "fe_render_workspace_bytes", "fe_render_fwd", "fe_profile_enable",
     "fe_profile_read", "fe_silhouette_workspace_bytes", "fe_silhouette_loss", "fe_adam_step",
     "fe_gmm_workspace_bytes", "fe_gmm_forward", "fe_gmm_backward", "fe_ray_flow",
-    "fe_rasterize_gaussians_fwd", "rasterize_gaussians_fwd",
+    "fe_rasterize_gaussians_fwd", "rasterize_gaussians_fwd", "fe_render_fwd_noalloc",
 ]
 KERNELS = ["preprocess", "mask1", "scan1", "write1", "blend1", "mask2", "scan2", "write2", "blend2",
            "blend_dense", "bwd_blend", "bwd_finalize"]
@@ -59,6 +59,7 @@ class FeOptions(ctypes.Structure):
 
 
 FE_FLAG_REUSE_DESCRIPTORS = 1
+FE_NOALLOC_SCRATCH_BYTES = 32 << 20
 
 
 class FeSilView(ctypes.Structure):
@@ -145,6 +146,10 @@ def load(path: str = LIB_PATH) -> ctypes.CDLL:
     lib.fe_ray_flow.restype = ctypes.c_int
     lib.fe_ray_flow.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_float, ctypes.c_int32,
                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    lib.fe_render_fwd_noalloc.restype = ctypes.c_int
+    lib.fe_render_fwd_noalloc.argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_void_p, ctypes.POINTER(FeCamera), ctypes.c_float, ctypes.c_float,
+                                          ctypes.c_void_p, ctypes.c_void_p]
     xla_args = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_char_p, ctypes.c_size_t]
     lib.fe_rasterize_gaussians_fwd.restype = ctypes.c_int
     lib.fe_rasterize_gaussians_fwd.argtypes = xla_args

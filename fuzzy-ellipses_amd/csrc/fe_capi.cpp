@@ -499,9 +499,77 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
   return FE_OK;
 }
 
+// Workspace of fe_render_fwd_noalloc: a static device array, one copy per GPU for the life of
+// the process.  Host code uses it only through the per-device address hipGetSymbolAddress
+// returns (the array's own name is a host-side stand-in, not a device address).
+static __device__ __align__(256) unsigned char scratch[FE_NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[64] = {};
+
+static fe_status scratch_address(void** out) {
+  int dev = 0;
+  FE_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_msg("fe_render_fwd_noalloc: device index out of range");
+    return FE_ERR_NO_DEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    FE_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      set_msg("fe_render_fwd_noalloc: no device address for the scratch array");
+      return FE_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return FE_OK;
+}
+
+fe_status render_fwd_noalloc(hipStream_t s, int32_t n, const float* means, const float* prec,
+                             const float* weights_log, const fe_camera* cam, float beta2, float beta3, float* depth,
+                             float* alpha) {
+  if (!cam) {
+    set_msg("fe_render_fwd_noalloc: null camera");
+    return FE_ERR_INVALID_ARGUMENT;
+  }
+  fe_scene sc{n, means, prec, weights_log};
+  fe_view v;
+  memset(&v, 0, sizeof(v));   // no device pose, no ray list
+  v.cam = *cam;
+  v.beta2 = beta2;
+  v.beta3 = beta3;
+  v.depth = depth;
+  v.alpha = alpha;
+  const fe_options o{16, 64, FE_MODE_CULLED, 0, 0, 0.0f, 0.0f};
+  Layout L;
+  fe_status st = plan(&sc, 1, &v, 1, &o, &L, nullptr);
+  if (st != FE_OK) return st;
+  if (L.bytes > (size_t)FE_NOALLOC_SCRATCH_BYTES) {
+    snprintf(g_err, sizeof(g_err),
+             "fe_render_fwd_noalloc: this view needs %zu bytes of scratch, more than the %zu the library holds "
+             "per GPU (FE_NOALLOC_SCRATCH_BYTES); use fe_render_fwd with a workspace",
+             L.bytes, (size_t)FE_NOALLOC_SCRATCH_BYTES);
+    return FE_ERR_WORKSPACE;
+  }
+  void* ws = nullptr;
+  st = scratch_address(&ws);
+  if (st != FE_OK) return st;
+  Job job;
+  return forward_impl(s, &sc, 1, &v, 1, &o, ws, (size_t)FE_NOALLOC_SCRATCH_BYTES, &job, true);
+}
+
 }  // namespace fe
 
 extern "C" {
+
+fe_status fe_render_fwd_noalloc(void* stream, int32_t n, const float* means, const float* prec,
+                                const float* weights_log, const fe_camera* cam, float beta2, float beta3,
+                                float* depth, float* alpha) {
+  return fe::render_fwd_noalloc(reinterpret_cast<hipStream_t>(stream), n, means, prec, weights_log, cam, beta2, beta3,
+                                depth, alpha);
+}
 
 fe_status fe_backward(void* stream, const fe_scene* scenes, int32_t n_scenes, const fe_view* views,
                       int32_t n_views, const fe_options* opt, const fe_view_cotangent* cot,

@@ -195,6 +195,22 @@ fe_status fe_render_fwd(void* stream, int32_t n, const float* means, const float
                         float beta3, float* depth, float* alpha, void* workspace,
                         size_t workspace_bytes);
 
+/* ---- fe_render_fwd without a workspace: allocates nothing ----
+ * No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the same render as
+ * fe_render_fwd (same arguments but the workspace, bit-identical images), with its workspace in a
+ * static device array of FE_NOALLOC_SCRATCH_BYTES that each GPU the library runs on holds until
+ * the process ends (32 MiB: enough for N = 4096 Gaussians at 480x640; a view that needs more is
+ * refused with FE_ERR_WORKSPACE, and fe_render_workspace_bytes tells the need).  The call makes no
+ * hipMalloc/hipFree or host staging allocation: the view's descriptors reach the device as a
+ * kernel argument.  Every call on one GPU shares that GPU's array, so calls that overlap on one
+ * GPU (different streams, or host threads) race; calls in order on one stream do not.  Nothing
+ * in the array needs clearing between calls: every call rewrites what it reads. */
+#define FE_NOALLOC_SCRATCH_BYTES (32u << 20)
+fe_status fe_render_fwd_noalloc(void* stream, int32_t n, const float* means, const float* prec,
+                                const float* weights_log, const fe_camera* cam, float beta2,
+                                float beta3, float* depth, float* alpha);
+
 /* ---- the reference's XLA custom-call target, unchanged in shape ----
  * Replaces RasterizeGaussiansCUDAJAX (rasterize_main.h:26-30, rasterize_main.cu:29-114),
  * which ext.cpp:22-32 exports as the capsule "rasterize_gaussians_fwd" and
