@@ -514,3 +514,47 @@ int64_t fe_ref_tiles_rects(int n, const float* rec, int any_degenerate, int64_t 
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// 4. Record order of one view (the renderer's k_order), restated from the inputs.  Gaussians
+//    are ordered within consecutive chunks of 1024 by (code, index), where code is a 20-bit
+//    Z-order (Morton) code of the projected centre: x = R p, the camera-frame position of
+//    p = mu - o (fma order x = R0 p0 + (R1 p1 + (R2 p2))), then for z > 0 the fractions
+//    u = clamp(sx (x/z) + ox, 0, 1) and v likewise over the window [-W/2, 3W/2] of a pixel grid
+//    (sx = fx / 2W, ox = (cx + W/2) / 2W), or [-2, 2] of the camera plane for a ray list
+//    (sx = 1/4, ox = 1/2); 10 bits each, trunc(u * 1023), x bits at even positions.  Centres
+//    not in front of the camera get the largest code.  A NaN fraction clamps to 0.
+// ------------------------------------------------------------------------------------------
+static uint32_t morton2(uint32_t ix, uint32_t iy) {
+  uint32_t c = 0;
+  for (int b = 0; b < 10; ++b) c |= (((ix >> b) & 1u) << (2 * b)) | (((iy >> b) & 1u) << (2 * b + 1));
+  return c;
+}
+
+extern "C" void fe_ref_order(int n, const float* means, const float* R9, const float* o3, float fx, float fy,
+                             float cx, float cy, int H, int W, int ray_view, int32_t* order) {
+  const float sx = ray_view ? 0.25f : fx / (2.0f * (float)W);
+  const float sy = ray_view ? 0.25f : fy / (2.0f * (float)H);
+  const float ox = ray_view ? 0.5f : (cx + 0.5f * (float)W) / (2.0f * (float)W);
+  const float oy = ray_view ? 0.5f : (cy + 0.5f * (float)H) / (2.0f * (float)H);
+  std::vector<std::pair<uint32_t, int32_t>> keyed((size_t)std::max(n, 0));
+  for (int g = 0; g < n; ++g) {
+    const float* mu = means + 3 * (size_t)g;
+    const float p0 = mu[0] - o3[0], p1 = mu[1] - o3[1], p2 = mu[2] - o3[2];
+    const float x = fmaf(R9[0], p0, fmaf(R9[1], p1, R9[2] * p2));
+    const float y = fmaf(R9[3], p0, fmaf(R9[4], p1, R9[5] * p2));
+    const float z = fmaf(R9[6], p0, fmaf(R9[7], p1, R9[8] * p2));
+    uint32_t code = 0xFFFFFu;
+    if (z > 0.0f) {
+      const float u = fminf(fmaxf(fmaf(sx, x / z, ox), 0.0f), 1.0f);
+      const float v = fminf(fmaxf(fmaf(sy, y / z, oy), 0.0f), 1.0f);
+      code = morton2((uint32_t)(u * 1023.0f), (uint32_t)(v * 1023.0f));
+    }
+    keyed[g] = {code, g};
+  }
+  for (int c0 = 0; c0 < n; c0 += 1024) {
+    const int c1 = std::min(n, c0 + 1024);
+    std::sort(keyed.begin() + c0, keyed.begin() + c1);   // (code, index): unique keys
+    for (int j = c0; j < c1; ++j) order[j] = keyed[j].second;
+  }
+}

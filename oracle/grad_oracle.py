@@ -43,15 +43,18 @@ def pinhole_rays(cam, pixels=None):
     return rc @ np.asarray(cam.R, dtype=np.float64)
 
 
-def grads(scene, cams, d_depth, d_alpha, chunk=2048):
+def grads(scene, cams, d_depth, d_alpha, chunk=2048, pixels=None):
     """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> (float64).
 
+    With ``pixels`` (one index array per view) only those pixels are rendered, and
+    d_depth[v] / d_alpha[v] hold their cotangents (the others being zero): the render is
+    separable per pixel, so this is the gradient of the full sum with zeros elsewhere.
     Returns (d_means (N,3), d_prec (N,3,3) -- upper triangle, d_weights_log (N,))."""
     means = torch.tensor(scene.means, dtype=torch.float64, requires_grad=True)
     prec = torch.tensor(scene.prec, dtype=torch.float64, requires_grad=True)
     w = torch.tensor(scene.weights_log, dtype=torch.float64, requires_grad=True)
     for v, cam in enumerate(cams):
-        rays_all = pinhole_rays(cam)
+        rays_all = pinhole_rays(cam, None if pixels is None else pixels[v])
         gd = np.asarray(d_depth[v], dtype=np.float64).reshape(-1)
         ga = np.asarray(d_alpha[v], dtype=np.float64).reshape(-1)
         for s in range(0, rays_all.shape[0], chunk):

@@ -60,6 +60,10 @@ def load():
     lib.fe_ref_tiles_rects.argtypes = [ctypes.c_int, _f32p, ctypes.c_int, ctypes.c_int64, _f32p, ctypes.c_float,
                                        ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                        ctypes.c_void_p, ctypes.c_void_p, _u32p, _u32p]
+    lib.fe_ref_order.restype = None
+    lib.fe_ref_order.argtypes = [ctypes.c_int, _f32p, _f32p, _f32p, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                 ctypes.c_float, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                 np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")]
     _lib = lib
     return lib
 
@@ -195,3 +199,21 @@ def tiles_rays(scene, R, t, xy, ts, tile_lmin=None, tile_seed=None):
                                    T, rects.reshape(-1), np.float32(scene.beta2), np.float32(scene.beta3),
                                    np.float32(c_alpha), np.float32(delta2), np.float32(floor_thr), lm, sd, counts, lst)
     return counts, lst[:total].copy()
+
+
+def order(scene, cam, ray_view=False):
+    """The renderer's record order of one view, restated from the inputs (fe_ref_order):
+    record j is Gaussian order[j].  For a ray-list view pass ray_view=True (cam needs R, t)."""
+    lib = load()
+    n = scene.n
+    out = np.zeros(max(n, 1), np.int32)
+    fx = 1.0 if ray_view else cam.fx
+    fy = 1.0 if ray_view else cam.fy
+    cx = 0.0 if ray_view else cam.cx
+    cy = 0.0 if ray_view else cam.cy
+    H = 1 if ray_view else cam.H
+    W = 1 if ray_view else cam.W
+    lib.fe_ref_order(n, np.ascontiguousarray(scene.means if n else np.zeros((1, 3)), np.float32).reshape(-1),
+                     np.ascontiguousarray(cam.R, np.float32).reshape(-1), np.ascontiguousarray(cam.t, np.float32),
+                     np.float32(fx), np.float32(fy), np.float32(cx), np.float32(cy), H, W, int(ray_view), out)
+    return out[:n]

@@ -70,7 +70,9 @@ def test_ray_list_keys_bit_exact(fea, scenes_mod):
     torch.cuda.synchronize()
     st = r.debug_state()
     from conftest import in_record_order
-    scene = in_record_order(scene, st["order"])
+    cpu_order = ref.order(scene, cam, ray_view=True)     # the record order, from the inputs
+    np.testing.assert_array_equal(st["order"], cpu_order)
+    scene = in_record_order(scene, cpu_order)
     off1, off2, lst = st["off1"], st["off2"], st["list"]
     c1, l1 = ref.tiles_rays(scene, cam.R, cam.t, xy, 16)
     np.testing.assert_array_equal(np.diff(off1), c1.astype(np.int64))

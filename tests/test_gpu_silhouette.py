@@ -131,6 +131,26 @@ def test_objective_kernel_matches_float64(fea):
         assert torch.allclose(dd.cpu().double(), dd64, rtol=1e-6, atol=0), "d_depth"
     print("objective: kernel %.9g float64 %.9g" % (losses[0], want))
     assert abs(losses[0] - want) <= 1e-5 * abs(want)
+    # the same objective from torch's own loss functions, cotangents by autograd
+    F = torch.nn.functional
+    lo, hi = float(np.float32(3e-8)), float(np.float32(1.0) - np.float32(3e-8))
+    total = 0.0
+    for (a, d, m, t, da, dd) in keep:
+        a64 = a.cpu().double().requires_grad_(True)
+        d64 = d.cpu().double().requires_grad_(True)
+        m64, t64 = m.cpu().double(), t.cpu().double()
+        bce = F.binary_cross_entropy(a64.clamp(lo, hi), m64, reduction="mean")
+        l1 = F.l1_loss(m64 * d64, m64 * t64, reduction="sum") / m64.sum().clamp_min(1.0)
+        lv = (bce + 0.1 * l1) / len(keep)
+        lv.backward()
+        total += float(lv.detach())
+        # at an alpha exactly on a clip bound torch's clamp passes the gradient, the kernel a zero
+        on_bound = (a64.detach() == lo) | (a64.detach() == hi)
+        ga = torch.where(on_bound, torch.zeros_like(a64.grad), a64.grad)
+        assert torch.allclose(da.cpu().double(), ga, rtol=1e-5, atol=1e-12), "d_alpha vs torch autograd"
+        assert torch.allclose(dd.cpu().double(), d64.grad, rtol=1e-6, atol=0), "d_depth vs torch autograd"
+    print("objective: torch.nn.functional %.9g" % total)
+    assert abs(losses[0] - total) <= 1e-5 * abs(total)
 
 
 def test_adam_kernel_matches_torch_optimizer(fea):

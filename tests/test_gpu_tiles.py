@@ -1,5 +1,6 @@
-"""Bit-exact tile keys on the GPU: per-Gaussian records, per-tile ranges and sorted
-Gaussian lists of both culling phases equal the CPU restatement (oracle/fe_ref.cpp)."""
+"""Bit-exact tile keys on the GPU: the record order, per-Gaussian records, per-tile ranges and
+sorted Gaussian lists of both culling phases equal the CPU restatement (oracle/fe_ref.cpp),
+computed from the inputs alone (phase 2 given the GPU's per-tile float thresholds)."""
 import numpy as np
 import pytest
 
@@ -15,8 +16,11 @@ def _check(fea, scene, cam, ts):
     r.forward([fea.SceneTensors.from_scene(scene)], [(0, cam, scene.beta2, scene.beta3)])
     torch.cuda.synchronize()
     st = r.debug_state()
-    # the GPU lists record ids; the restatement runs on the Gaussians in that order
-    scene = in_record_order(scene, st["order"])
+    # the record order (Z order of projected centres in chunks of 1024), restated on the CPU
+    cpu_order = ref.order(scene, cam)
+    np.testing.assert_array_equal(st["order"], cpu_order)
+    # the GPU lists record ids; the restatement runs on the Gaussians in the CPU's order
+    scene = in_record_order(scene, cpu_order)
     rec_cpu, _ = ref.records(scene, cam)
     np.testing.assert_array_equal(st["records"].view(np.uint32), rec_cpu.view(np.uint32))
     off1, off2, lst = st["off1"], st["off2"], st["list"]

@@ -4,6 +4,8 @@ Phase 1 must list every (pixel, Gaussian) pair whose density reaches exp(-c_alph
 union of phase 1 and 2 must list every pair whose depth logit is within the depth margin of
 the pixel's log-sum-exp.  Both are checked against float64 values of every pair.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -173,3 +175,55 @@ def test_masked_logit_bound_holds(scenes_mod, kind, ts):
     bad, checked = ref.masked_bound_check(scene, cam, ts)
     print(f"{kind} ts={ts}: {checked} masked (pixel, Gaussian) logits checked, {bad} violating pairs")
     assert checked > 0 and bad == 0
+
+
+def _morton_f64(scene, cam, ray_view=False):
+    """Z-order code of every projected centre in float64 (an independent statement of the
+    record-order key: 10 bits per axis over [-W/2, 3W/2] x [-H/2, 3H/2] pixels, x bits even),
+    with the distance of each scaled fraction to the nearest integer (where float32 rounding
+    may land on the other side)."""
+    p = (np.asarray(scene.means, np.float64) - np.asarray(cam.t, np.float64)) @ np.asarray(cam.R, np.float64).T
+    z = p[:, 2]
+    with np.errstate(divide="ignore", invalid="ignore"):
+        if ray_view:
+            u, v = (p[:, 0] / z + 2.0) / 4.0, (p[:, 1] / z + 2.0) / 4.0
+        else:
+            u = (cam.fx * p[:, 0] / z + cam.cx + cam.W / 2) / (2 * cam.W)
+            v = (cam.fy * p[:, 1] / z + cam.cy + cam.H / 2) / (2 * cam.H)
+    su, sv = np.clip(u, 0, 1) * 1023.0, np.clip(v, 0, 1) * 1023.0
+    iu, iv = np.floor(su).astype(np.int64), np.floor(sv).astype(np.int64)
+    code = np.zeros(scene.n, np.int64)
+    for b in range(10):
+        code |= ((iu >> b) & 1) << (2 * b) | ((iv >> b) & 1) << (2 * b + 1)
+    code[~(z > 0)] = 0xFFFFF
+    edge = np.minimum(np.abs(su - np.rint(su)), np.abs(sv - np.rint(sv)))
+    return code, np.where(z > 0, edge, 1.0)
+
+
+@pytest.mark.parametrize("which", ["shell", "single", "rays", "inside"])
+def test_record_order_restatement(scenes_mod, which):
+    """fe_ref_order (the renderer's record order) against the float64 statement: each chunk of
+    1024 Gaussians is a permutation of its own indices, sorted by code then index, wherever the
+    float64 code is not within rounding of a cell edge."""
+    if which == "inside":     # camera inside a cube: centres behind it get the last code
+        scene = scenes_mod.single_view_scene(seed=4, n=2500)
+        cam = dataclasses.replace(scene.cameras[0], t=np.array([0.05, -0.02, 0.1]))
+    elif which == "single":
+        scene = scenes_mod.single_view_scene(seed=1, n=3000)
+        cam = scene.cameras[0]
+    else:
+        scene = scenes_mod.shell_scene()
+        cam = scene.cameras[3]
+    order = ref.order(scene, cam, ray_view=(which == "rays"))
+    code, edge = _morton_f64(scene, cam, ray_view=(which == "rays"))
+    n = scene.n
+    sure = edge > 1e-4
+    for c0 in range(0, n, 1024):
+        o = order[c0:c0 + 1024]
+        np.testing.assert_array_equal(np.sort(o), np.arange(c0, min(n, c0 + 1024)))
+        key = code[o] * 4096 + o
+        ok = sure[o]
+        ks = key[ok]
+        assert np.all(np.diff(ks) > 0), "record order is not ascending in (code, index)"
+    if which == "inside":
+        assert np.any(code == 0xFFFFF) and np.any(code < 0xFFFFF)   # both kinds present
