@@ -30,19 +30,57 @@ TRANS_PEAK_TOPS = 9.8             # 256 CUs x 4 SIMDs x 16 lanes / 4 (quarter ra
 
 
 
+HBM_PEAK_GBS = 8000.0             # MI355X published HBM3E bandwidth
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "traffic_shell_n4096_16x480x640.json")
+PROFILE_NAMES = {"bwd_blend": "fe::k_bwd_blend<16>", "blend1": "fe::k_blend<16, 64, 1>",
+                 "blend2": "fe::k_blend<16, 64, 2>"}
 
 
-def measured_traffic(dom, args):
-    """HBM bytes per launch of kernel `dom` (FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a copy kernel)
-    from the committed rocprofv3 --pmc summary of this workload; None for a configuration it does not cover."""
+def profiled_counters(dom, args):
+    """Per-dispatch counters of kernel `dom` on this workload from the committed rocprofv3 --pmc
+    summary (HBM bytes: FETCH_SIZE x2 + WRITE_SIZE x1, calibrated on a copy kernel; VALU wave
+    instructions), or None for a configuration it does not cover."""
     if args.tile != 16 or args.batch != 64 or args.mode != "culled" or not os.path.exists(TRAFFIC_FILE):
         return None
-    names = {"bwd_blend": "fe::k_bwd_blend<16>", "blend1": "fe::k_blend<16, 64, 1>", "blend2": "fe::k_blend<16, 64, 2>"}
     with open(TRAFFIC_FILE) as f:
-        k = json.load(f)["kernels"].get(names.get(dom, ""))
-    return None if k is None else int(k["read_bytes"] + k["write_bytes"])
+        return json.load(f)["kernels"].get(PROFILE_NAMES.get(dom, ""))
+
+
+def roofline(dom, avg_s, pairs, flops_per_pair, alg_bytes, args):
+    """Roofline record of one kernel: algorithmic FP32 flops (the hot-path spec's per-pair figure)
+    and HBM bytes against MI355X's peaks, with the issued-instruction view beside them."""
+    achieved = flops_per_pair * pairs / avg_s / 1e12 if avg_s > 0 else 0.0
+    trans = TRANS_PER_PAIR * pairs / avg_s / 1e12 if avg_s > 0 else 0.0
+    prof = profiled_counters(dom, args)
+    traffic = None if prof is None else int(prof["read_bytes"] + prof["write_bytes"])
+    out = {"bound": "fp32_valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+           "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic, "kernel": dom,
+           "launch_avg_ms": round(avg_s * 1e3, 5), "pairs_per_launch": int(pairs),
+           "flops_per_pair": flops_per_pair,
+           "flops_model": "algorithmic FP32 flops per evaluated (pixel, Gaussian) pair, FMA = 2 (no MFMA: "
+                          "per-ray 3x3 forms are not a dense contraction)",
+           "trans_achieved": round(trans, 3), "trans_peak": TRANS_PEAK_TOPS, "trans_unit": "Top/s",
+           "trans_frac": round(trans / TRANS_PEAK_TOPS, 4),
+           "algorithmic_bytes": int(alg_bytes),
+           "hbm_algorithmic_gbs": round(alg_bytes / avg_s / 1e9, 2) if avg_s > 0 else 0.0,
+           "hbm_peak_gbs": HBM_PEAK_GBS}
+    if traffic is not None:
+        out.update(traffic_source=os.path.relpath(TRAFFIC_FILE, ROOT) + " (rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE "
+                   "per dispatch of this workload, copy-calibrated)",
+                   hbm_achieved_gbs=round(traffic / avg_s / 1e9, 2),
+                   hbm_frac=round(traffic / avg_s / 1e9 / HBM_PEAK_GBS, 4),
+                   traffic_over_algorithmic=round(traffic / alg_bytes, 3))
+    if prof is not None and "valu_insts" in prof:
+        vpp = prof["valu_insts"] * 64 / pairs
+        issued = 2.0 * prof["valu_insts"] * 64 / avg_s / 1e12
+        out.update(issued_valu_per_pair=round(vpp, 2),
+                   issued_flops_upper_tflops=round(issued, 3),
+                   issued_frac_upper=round(issued / FP32_PEAK_TFLOPS, 4),
+                   issued_note="SQ_INSTS_VALU x 64 lanes per evaluated pair from the same profile; the upper "
+                               "bound counts every issued VALU instruction as an FMA (2 flops per lane)")
+    return out
+
 
 def _package():
     if "fuzzy_ellipses_amd" in sys.modules:
@@ -115,16 +153,17 @@ def served(fea, sc, dev, rate=800.0, n=1000, burst=False, max_batch=64, slots=2,
     return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()}
 
 
-def workload_batched(fea, sc, dev, steps):
-    """256 scenes (N ~ U{16..2048}) with 1-8 views each at mixed resolutions, one call per
-    kernel for the whole batch, vs one call per view."""
+def batched_summary(fea, sc, dev, steps, tune_path=None):
+    """256 scenes (N ~ U{16..2048}) with 1-8 views each at mixed resolutions: one call per
+    kernel for the whole batch (autotuned tile and LDS batch), the same call replayed from a
+    HIP graph, and one call per view (the plain path)."""
     import torch
     from fuzzy_ellipses_amd.autotune import Autotuner
     scenes = sc.batched_scenes()
     sts = [fea.SceneTensors.from_scene(s, dev) for s in scenes]
     views = [(i, c, s.beta2, s.beta3) for i, s in enumerate(scenes) for c in s.cameras]
-    at = Autotuner(dev, reps=2)
-    tile, batch = at.tune(sts, views, backward=False)
+    at = Autotuner(dev, path=tune_path, reps=2)
+    tile, batch = at.tune(sts, views, backward=False, force=True)
     r = fea.Renderer(tile=tile, batch=batch, device=dev)
     outs = r.alloc_outputs(views)
     r.forward(sts, views, outs=outs)
@@ -162,48 +201,64 @@ def workload_batched(fea, sc, dev, steps):
     torch.cuda.synchronize()
     el_1 = time.perf_counter() - t0
     dense = sum(sts[si].n * cam.H * cam.W for (si, cam, _, _) in views)
-    return {"metric": "batched forward frames/s (256 scenes, mixed N and resolution)",
-            "value": round(len(views) / el, 2), "unit": "frames/s", "n_gpus": 1, "steps": steps,
-            "ms_per_step": round(1e3 * el, 3), "higher_is_better": True, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": "256 scenes N~U{16..2048}, 1-8 views each, resolutions 64^2..480x640",
-                       "views": len(views), "tile": tile, "lds_batch": batch},
+    key = next(iter(at.table)) if at.table else None
+    return {"workload": "256 scenes N~U{16..2048}, 1-8 views each, resolutions 64^2..480x640, forward",
+            "views": len(views), "tile": tile, "lds_batch": batch, "steps": steps,
+            "frames_per_s": round(len(views) / el, 2), "ms_per_call": round(1e3 * el, 3),
             "graph_replay_frames_per_s": round(len(views) / el_g, 2),
             "plain_one_view_per_call_frames_per_s": round(len(views) / el_1, 2),
             "speedup_vs_plain": round(el_1 / el, 2),
             "evals_per_s": stats["evals"] / el, "dense_equiv_evals_per_s": dense / el,
-            "autotune_sweep_ms": at.table.get(next(iter(at.table)), {}).get("sweep") if at.table else None}
+            "autotune_sweep_ms": at.table[key]["sweep"] if key else None}
 
 
-def workload_serving(fea, sc, dev, rates, n_per_rate):
-    """Poisson request mix (70% forward, 30% forward+backward) over 64 resident scenes, plus
-    the saturated fwd+bwd throughput at 480x640, N=4096 of the batched server vs the plain
-    one-request-per-launch path."""
-    out = {"metric": "served p50/p99 ms and throughput", "unit": "ms", "n_gpus": 1, "data": "synthetic",
-           "config": {"workload": "64 resident scenes N~U{64..4096}, resolutions 128^2/240x320/480x640, "
-                                  "70% fwd / 30% fwd+bwd, Poisson arrivals"}, "rates": {}}
+def workload_batched(fea, sc, dev, steps):
+    b = batched_summary(fea, sc, dev, steps)
+    return {"metric": "batched forward frames/s (256 scenes, mixed N and resolution)", "value": b["frames_per_s"],
+            "unit": "frames/s", "n_gpus": 1, "steps": steps, "ms_per_step": b["ms_per_call"],
+            "higher_is_better": True, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": b["workload"], "views": b["views"], "tile": b["tile"], "lds_batch": b["lds_batch"]},
+            **{k: v for k, v in b.items() if k not in ("workload", "views", "tile", "lds_batch", "steps")}}
+
+
+def serving_summary(fea, sc, dev, rates, n_per_rate):
+    """Poisson request mix (70% forward, 30% forward+backward) over 64 resident scenes at each
+    rate, beside the plain one-request-per-launch path at the same rate; the saturated
+    throughput of the batched server vs the plain path for fwd+bwd at 480x640, N=4096 and for
+    the request mix itself (every request queued at once)."""
+    out = {"workload": "64 resident scenes N~U{64..4096}, resolutions 128^2/240x320/480x640, 70% fwd / "
+                       "30% fwd+bwd, Poisson arrivals", "rates": {}}
     for rate, n in zip(rates, n_per_rate):
-        out["rates"][str(rate)] = served(fea, sc, dev, rate=rate, n=n)
-        plain = served(fea, sc, dev, rate=rate, n=max(n // 4, 50), max_batch=1, slots=1)
-        out["rates"][str(rate)]["plain_p50_ms"] = plain["p50_ms"]
-        out["rates"][str(rate)]["plain_p99_ms"] = plain["p99_ms"]
-        out["rates"][str(rate)]["plain_throughput_rps"] = plain["throughput_rps"]
+        e = served(fea, sc, dev, rate=rate, n=n)
+        plain = served(fea, sc, dev, rate=rate, n=max(n // 3, 50), max_batch=1, slots=1)
+        e.update(plain_requests=plain["requests"], plain_p50_ms=plain["p50_ms"], plain_p99_ms=plain["p99_ms"],
+                 plain_throughput_rps=plain["throughput_rps"])
+        out["rates"][str(int(rate)) if float(rate).is_integer() else str(rate)] = e
     shell = sc.shell_scene(n_views=1)
     big = [sc.Scene(shell.means, shell.prec, shell.weights_log, [])]
     sat_b = served(fea, sc, dev, rate=1.0, n=160, burst=True, max_batch=16, slots=2, resolutions=((480, 640),),
                    backward_frac=1.0, scenes=big)
     sat_p = served(fea, sc, dev, rate=1.0, n=40, burst=True, max_batch=1, slots=1, resolutions=((480, 640),),
                    backward_frac=1.0, scenes=big)
-    out["saturated_fwd_bwd_480x640_n4096"] = {"batched_rps": sat_b["throughput_rps"],
-                                              "plain_rps": sat_p["throughput_rps"],
+    out["saturated_fwd_bwd_480x640_n4096"] = {"requests": 160, "batched_rps": sat_b["throughput_rps"],
+                                              "plain_requests": 40, "plain_rps": sat_p["throughput_rps"],
                                               "ratio": round(sat_b["throughput_rps"] / sat_p["throughput_rps"], 3)}
-    # the request mix itself with every request queued at once (throughput-bound)
     mix_b = served(fea, sc, dev, rate=1.0, n=2000, burst=True, max_batch=256)
     mix_p = served(fea, sc, dev, rate=1.0, n=400, burst=True, max_batch=1, slots=1)
-    out["saturated_mix"] = {"batched_rps": mix_b["throughput_rps"], "batched_p50_ms": mix_b["p50_ms"],
+    out["saturated_mix"] = {"requests": 2000, "batched_rps": mix_b["throughput_rps"], "batched_p50_ms": mix_b["p50_ms"],
                             "batched_p99_ms": mix_b["p99_ms"], "mean_batch": mix_b["mean_batch"],
-                            "plain_rps": mix_p["throughput_rps"],
+                            "plain_requests": 400, "plain_rps": mix_p["throughput_rps"],
                             "ratio": round(mix_b["throughput_rps"] / mix_p["throughput_rps"], 3)}
-    out["value"] = out["rates"][str(rates[-1])]["p99_ms"]
+    out["target_20x_met"] = bool(out["saturated_fwd_bwd_480x640_n4096"]["ratio"] >= 20.0)
+    return out
+
+
+def workload_serving(fea, sc, dev, rates, n_per_rate):
+    out = {"metric": "served p50/p99 ms and throughput", "unit": "ms", "n_gpus": 1, "data": "synthetic"}
+    s = serving_summary(fea, sc, dev, rates, n_per_rate)
+    out["config"] = {"workload": s.pop("workload")}
+    out.update(s)
+    out["value"] = out["rates"][list(out["rates"])[-1]]["p99_ms"]
     out["higher_is_better"] = False
     return out
 
@@ -278,8 +333,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="shell", choices=["shell", "batched", "serving", "silhouette", "gmm"])
     ap.add_argument("--rates", default="50,200,800")
-    ap.add_argument("--requests", default="500,1000,2000")
+    ap.add_argument("--requests", default="300,600,1200")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (calibrates HBM byte counters)")
     args = ap.parse_args()
@@ -379,24 +435,23 @@ def main():
     f_fps = n_views * args.steps / el_f
 
     # roofline of the dominant kernel, its launches timed over the timed region
-    flops_per_launch = {
-        "blend1": FLOPS_FWD * stats["evals_phase1"],
-        "blend2": FLOPS_FWD * (evals - stats["evals_phase1"]),
-        "blend_dense": FLOPS_FWD * evals,
-        "bwd_blend": FLOPS_BWD * evals,
-    }
+    pairs = {"blend1": stats["evals_phase1"], "blend2": evals - stats["evals_phase1"], "blend_dense": evals,
+             "bwd_blend": evals}
+    n_rec = scene.n * n_views
+    K = stats["pairs_phase1"] + stats["pairs_phase2"]
+    # algorithmic HBM bytes: per pixel 8 B out + 16 B saved aggregates (forward) or 16 B
+    # aggregates + 8 B cotangents read (backward); 4 B per listed pair; per (Gaussian, view)
+    # the backward reads the 64-B record and writes its 11 sums (44 B)
+    alg_bytes = {"blend1": 24 * P * n_views + 4 * stats["pairs_phase1"],
+                 "blend2": 24 * P * n_views + 4 * stats["pairs_phase2"],
+                 "blend_dense": 24 * P * n_views,
+                 "bwd_blend": 24 * P * n_views + 4 * K + 108 * n_rec}
     kavg = {k: (ms / n if n else 0.0) for k, (ms, n) in kt_all.items()}
     avg_s = (kt[dom][0] / kt[dom][1] if kt[dom][1] else 0.0) / 1e3
-    achieved = flops_per_launch[dom] / avg_s / 1e12 if avg_s > 0 else 0.0
-    pairs_dom = flops_per_launch[dom] / (FLOPS_BWD if dom == "bwd_blend" else FLOPS_FWD)
-    trans = TRANS_PER_PAIR * pairs_dom / avg_s / 1e12 if avg_s > 0 else 0.0
-    roof = {"bound": "mfma", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "launch_avg_ms": round(avg_s * 1e3, 5),
-            "trans_achieved": round(trans, 3), "trans_peak": TRANS_PEAK_TOPS, "trans_unit": "Top/s",
-            "trans_frac": round(trans / TRANS_PEAK_TOPS, 4),
-            "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": measured_traffic(dom, args), "kernel": dom,
-            "note": "compute-bound on FP32 VALU (no MFMA, no packed FP32); flops/launch = "
-                    f"{FLOPS_FWD if dom != 'bwd_blend' else FLOPS_BWD} x evaluated pairs"}
+    roof = roofline(dom, avg_s, pairs[dom], FLOPS_BWD if dom == "bwd_blend" else FLOPS_FWD, alg_bytes[dom], args)
+    fwd_k = "blend1" if args.mode == "culled" else "blend_dense"
+    roof_fwd = roofline(fwd_k, kavg[fwd_k] / 1e3, pairs[fwd_k], FLOPS_FWD, alg_bytes[fwd_k], args)
+    roof_fwd["timing"] = "every launch event-timed in the breakdown loop before the timed region"
 
     out = {
         "metric": METRIC, "value": round(fb_fps, 3), "unit": "frames/s", "n_gpus": 1, "steps": args.steps,
@@ -413,9 +468,14 @@ def main():
         "pairs_listed": stats["pairs_phase1"] + stats["pairs_phase2"], "floor_tiles": stats["floor_tiles"],
         "kernel_avg_ms": {k: round(v, 5) for k, v in kavg.items() if kt_all[k][1]},
         "roofline": roof,
+        "roofline_forward_blend": roof_fwd,
     }
+    if not args.no_batched:
+        out["batched"] = batched_summary(fea, sc, dev, steps=max(args.steps // 2, 5))
     if not args.no_serving:
-        out["served"] = served(fea, sc, dev, rate=800.0, n=1000)
+        rates = [float(x) for x in args.rates.split(",")]
+        nreq = [int(x) for x in args.requests.split(",")]
+        out["served"] = serving_summary(fea, sc, dev, rates, nreq)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(scene, cams[0])
         out["cpu_forward_f32"] = cpu_forward_f32(scene, cams[0])

@@ -152,6 +152,13 @@ struct Counters {
   uint32_t tile_hist[2][kTileBuckets];   // dispatch buckets: tiles per bucket, then bucket starts
 };
 
+// Whether the membership bounds apply to a view: they bound the logit b3 d2 - b2 t from above
+// by the least distance and the least depth over a tile, which needs finite b2 >= 0 and
+// b3 >= 0.  Other views list every pair (like views with degenerate records).
+__device__ __forceinline__ bool bounds_apply(const Job& jb) {
+  return jb.beta3 >= 0.0f && jb.beta3 <= 3.0e38f && jb.beta2 >= 0.0f && jb.beta2 <= 3.0e38f;
+}
+
 // Whether a view's blends use the blend records: no degenerate record, every record valid,
 // and b3 > 0 (the running maximum is taken over l/b3).
 __device__ __forceinline__ bool use_rec2(const Job& jb, int32_t flags) {

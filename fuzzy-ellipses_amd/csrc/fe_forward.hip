@@ -350,7 +350,7 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
     }
   }
   __syncthreads();
-  const bool dense_job = (job_flags[ji] & 1) != 0;
+  const bool dense_job = (job_flags[ji] & 1) != 0 || !bounds_apply(jb);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
   if (PHASE == 2 && !dense_job) {

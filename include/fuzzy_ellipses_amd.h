@@ -66,7 +66,8 @@ typedef struct fe_scene {
 typedef struct fe_view {
   int32_t scene;   /* index into the scene array */
   fe_camera cam;
-  float beta2, beta3;
+  float beta2, beta3;  /* culling bounds need finite beta2 >= 0 and beta3 >= 0; a view with
+                          other values is rendered with every pair listed (same results) */
   float* depth;    /* (H,W) device, written by fe_forward */
   float* alpha;    /* (H,W) device, written by fe_forward */
   const float* pose_device; /* optional device array R[9] (row-major) then t[3]; when set it

@@ -320,6 +320,8 @@ static int64_t tiles_impl(int n, const float* rec, int any_degenerate, int tx, i
                           GroupGeoT ggeo, float beta2, float beta3, float c_alpha, float delta2, float floor_thr,
                           const float* tile_lmin, const float* tile_seed, uint32_t* counts, uint32_t* list) {
   const float b3l = beta3 * kLog2e, b2l = beta2 * kLog2e;
+  // the bounds hold for finite beta2 >= 0 and beta3 >= 0; other views list every pair
+  if (!(beta3 >= 0.0f && beta3 <= 3.0e38f && beta2 >= 0.0f && beta2 <= 3.0e38f)) any_degenerate = 1;
   int64_t pos = 0;
   std::vector<char> in1(n > 0 ? n : 1, 0);
   // phase-1 thresholds, with the bound-based replacement for tiles whose value is -inf

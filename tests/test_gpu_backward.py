@@ -95,3 +95,53 @@ def test_backward_weight_floor(fea, scenes_mod, mode, centre):
     errs = [_rel(g, r) for g, r in zip(got, ref)]
     print(f"backward weight floor {centre} {mode}: rel err means {errs[0]:.2e} prec {errs[1]:.2e} w {errs[2]:.2e}")
     assert max(errs) <= GRAD_RTOL
+
+
+def _direct_form_case(scenes_mod, case):
+    """Scenes whose views take the direct-form forward and backward (no blend records): a
+    Gaussian with a zero diagonal entry of U (a degenerate record: the view is rendered densely
+    with the NaN-safe blend), the camera at a Gaussian's centre (|q|^2 = 0), beta3 = 0 and
+    beta3 < 0 (the blend records' running maximum needs beta3 > 0)."""
+    import dataclasses
+    scene = scenes_mod.single_view_scene(seed=14)
+    cam = scene.cameras[0]
+    if case == "zero_diagonal":
+        prec = scene.prec.copy()
+        prec[5, 1, 1] = 0.0
+        return dataclasses.replace(scene, prec=prec), cam
+    if case == "camera_at_centre":
+        # a camera centre exact in float32, so the oracle and the GPU both see q = 0 exactly
+        cam = dataclasses.replace(cam, t=np.asarray(cam.t, np.float32).astype(np.float64))
+        means = scene.means.copy()
+        means[9] = cam.t
+        return dataclasses.replace(scene, means=means, cameras=[cam]), cam
+    if case == "beta3_zero":
+        return dataclasses.replace(scene, beta3=0.0), cam
+    return dataclasses.replace(scene, beta3=-0.5), cam
+
+
+@pytest.mark.parametrize("mode", ["dense", "culled"])
+@pytest.mark.parametrize("case", ["zero_diagonal", "camera_at_centre", "beta3_zero", "beta3_negative"])
+def test_backward_direct_form(fea, scenes_mod, case, mode):
+    scene, cam = _direct_form_case(scenes_mod, case)
+    rng = np.random.default_rng(35)
+    gd = rng.normal(size=cam.H * cam.W)
+    ga = rng.normal(size=cam.H * cam.W) * 10
+    ref = go.grads(scene, [cam], [gd], [ga])
+    assert all(np.all(np.isfinite(r)) for r in ref), "the reference gradient should be finite here"
+    got = _gpu_grads(fea, scene, [cam], [gd], [ga], mode)
+    assert all(np.all(np.isfinite(g)) for g in got)
+    # the forward of the same view against the FP64 oracle
+    import torch
+    from oracle import ref as oref
+    r = fea.Renderer(tile=16, batch=64, mode=mode)
+    (d, a), = r.forward([fea.SceneTensors.from_scene(scene)], [(0, cam, scene.beta2, scene.beta3)])
+    torch.cuda.synchronize()
+    od, oa = oref.render_f64(scene, cam)
+    ed = float(np.max(np.abs(d.cpu().numpy().ravel() - od) / np.abs(od)))
+    ea = float(np.max(np.abs(a.cpu().numpy().ravel() - oa)))
+    print(f"forward direct form {case} {mode}: depth rel {ed:.2e}, alpha abs {ea:.2e}")
+    assert ed <= 1e-4 and ea <= 1e-4
+    errs = [_rel(g, r) for g, r in zip(got, ref)]
+    print(f"backward direct form {case} {mode}: rel err means {errs[0]:.2e} prec {errs[1]:.2e} w {errs[2]:.2e}")
+    assert max(errs) <= GRAD_RTOL

@@ -89,3 +89,14 @@ def test_keys_and_order_on_81920_tiles(fea, scenes_mod):
     assert (cam.H // 8) * (cam.W // 8) > 65536
     k1, k2 = _check(fea, scene, cam, 8)
     print(f"2048x2560 tile 8: phase-1 pairs {k1}, phase-2 pairs {k2}")
+
+
+@pytest.mark.parametrize("betas", [(21.4, -0.5), (-3.0, 2.66)])
+def test_keys_betas_outside_the_bounds(fea, scenes_mod, betas):
+    """A view with beta3 < 0 or beta2 < 0 lists every pair (the bounds assume both >= 0), on
+    the GPU and in the restatement."""
+    import dataclasses
+    scene = scenes_mod.single_view_scene(seed=6)
+    scene = dataclasses.replace(scene, beta2=betas[0], beta3=betas[1])
+    k1, k2 = _check(fea, scene, scene.cameras[0], 16)
+    assert k1 == scene.n * 16 and k2 == 0
