@@ -31,7 +31,7 @@ from .scenes import BETA2, BETA3, Camera, Scene
 __all__ = [
     "Renderer", "SceneTensors", "render_camera", "render_func_quat", "rasterize", "quat_to_rot",
     "prec_from_scale_quat", "FeError", "Camera", "Scene", "BETA2", "BETA3", "RayList",
-    "render_func_idx_quattrans", "render_func_idx_quattrans_flow",
+    "render_func_idx_quattrans", "render_func_idx_quattrans_flow", "render_features",
 ]
 
 MODES = {"culled": _lib.FE_MODE_CULLED, "dense": _lib.FE_MODE_DENSE}
@@ -46,9 +46,11 @@ def _stream(device: torch.device) -> int:
 
 
 class SceneTensors:
-    """Gaussians of one scene resident on the device (float32, contiguous)."""
+    """Gaussians of one scene resident on the device (float32, contiguous), optionally with
+    per-Gaussian features (N, C) (colour, descriptors, ...): the views of such a scene also
+    render the features weighted by render_func_rays' depth weights (fm_render.py:96-109)."""
 
-    def __init__(self, means, prec, weights_log, device="cuda"):
+    def __init__(self, means, prec, weights_log, device="cuda", features=None):
         dev = torch.device(device)
         self.means = torch.as_tensor(means, dtype=torch.float32, device=dev).contiguous()
         self.prec = torch.as_tensor(prec, dtype=torch.float32, device=dev).contiguous()
@@ -56,17 +58,27 @@ class SceneTensors:
         n = self.means.shape[0]
         if self.means.shape != (n, 3) or self.prec.shape != (n, 3, 3) or self.weights_log.shape != (n,):
             raise ValueError("SceneTensors: expected means (N,3), prec (N,3,3), weights_log (N,)")
+        self.features = None
+        if features is not None:
+            self.features = torch.as_tensor(features, dtype=torch.float32, device=dev).contiguous()
+            if self.features.dim() != 2 or self.features.shape[0] != n or not 1 <= self.features.shape[1] <= 65536:
+                raise ValueError("SceneTensors: expected features (N, C) with 1 <= C <= 65536")
 
     @classmethod
-    def from_scene(cls, scene: Scene, device="cuda") -> "SceneTensors":
-        return cls(scene.means, scene.prec, scene.weights_log, device)
+    def from_scene(cls, scene: Scene, device="cuda", features=None) -> "SceneTensors":
+        return cls(scene.means, scene.prec, scene.weights_log, device, features=features)
 
     @property
     def n(self) -> int:
         return int(self.means.shape[0])
 
+    @property
+    def n_features(self) -> int:
+        return 0 if self.features is None else int(self.features.shape[1])
+
     def fe(self) -> FeScene:
-        return FeScene(self.n, _ptr(self.means), _ptr(self.prec), _ptr(self.weights_log))
+        return FeScene(self.n, _ptr(self.means), _ptr(self.prec), _ptr(self.weights_log), _ptr(self.features),
+                       self.n_features)
 
 
 class RayList:
@@ -111,11 +123,16 @@ def fe_camera(cam) -> FeCamera:
     return c
 
 
-def _fe_view(si, cam, b2, b3, depth_ptr, alpha_ptr) -> FeView:
+def _fe_view(si, cam, b2, b3, depth_ptr, alpha_ptr, feat_ptr=None) -> FeView:
     v = FeView(int(si), fe_camera(cam), float(b2), float(b3), depth_ptr, alpha_ptr)
     if isinstance(cam, RayList):
         v.rays, v.n_rays = cam.xy.data_ptr(), cam.n
+    v.features_out = feat_ptr
     return v
+
+
+def _n_pix(cam) -> int:
+    return cam.n if isinstance(cam, RayList) else int(cam.H) * int(cam.W)
 
 
 class Renderer:
@@ -159,22 +176,25 @@ class Renderer:
         return (int(si), ck, float(np.float32(b2)), float(np.float32(b3)))
 
     def _structs(self, scenes: Sequence[SceneTensors], views, outs):
-        key = (tuple((s.n, _ptr(s.means), _ptr(s.prec), _ptr(s.weights_log)) for s in scenes),
-               tuple(self._view_key(v) for v in views), tuple((_ptr(d), _ptr(a)) for d, a in outs))
+        key = (tuple((s.n, _ptr(s.means), _ptr(s.prec), _ptr(s.weights_log), _ptr(s.features), s.n_features)
+                     for s in scenes),
+               tuple(self._view_key(v) for v in views), tuple(tuple(_ptr(x) for x in o) for o in outs))
         if self._cache is not None and self._cache[0] == key:
             return self._cache[1], self._cache[2]
         sc = (FeScene * len(scenes))(*[s.fe() for s in scenes])
         vw = (FeView * len(views))()
         for i, (si, cam, b2, b3) in enumerate(views):
-            d, a = outs[i]
-            vw[i] = _fe_view(si, cam, b2, b3, _ptr(d), _ptr(a))
+            o = outs[i]
+            vw[i] = _fe_view(si, cam, b2, b3, _ptr(o[0]), _ptr(o[1]), _ptr(o[2]) if len(o) > 2 else None)
         self._cache = (key, sc, vw)
         return sc, vw
 
-    def _check_image(self, t, cam, what):
-        """A per-view image buffer: contiguous float32 on the render device with one value per
-        pixel (H*W) or ray (n); the kernels index it by pixel."""
-        n = cam.n if isinstance(cam, RayList) else int(cam.H) * int(cam.W)
+    def _check_image(self, t, cam, what, channels=1):
+        """A per-view image buffer: contiguous float32 on the render device with `channels`
+        values per pixel (H*W) or ray (n); the kernels index it by pixel."""
+        n = _n_pix(cam) * channels
+        if channels < 1:
+            raise FeError(f"{what}: the view's scene has no features")
         if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_contiguous() \
                 or t.device != self.device or t.numel() != n:
             raise FeError(f"{what}: expected a contiguous float32 tensor of {n} values on {self.device}")
@@ -196,11 +216,18 @@ class Renderer:
             self._ws = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=self.device)
         return self._ws
 
-    def alloc_outputs(self, views):
+    def alloc_outputs(self, views, scenes=None):
+        """(depth, alpha) per view, plus the feature image (.., C) for views whose scene has
+        features (when ``scenes`` is given)."""
         shape = lambda cam: (cam.n,) if isinstance(cam, RayList) else (cam.H, cam.W)
-        return [(torch.empty(shape(cam), dtype=torch.float32, device=self.device),
+        out = []
+        for (si, cam, _, _) in views:
+            o = (torch.empty(shape(cam), dtype=torch.float32, device=self.device),
                  torch.empty(shape(cam), dtype=torch.float32, device=self.device))
-                for (_, cam, _, _) in views]
+            if scenes is not None and scenes[si].n_features > 0:
+                o = o + (torch.empty(shape(cam) + (scenes[si].n_features,), dtype=torch.float32, device=self.device),)
+            out.append(o)
+        return out
 
     # -- forward ------------------------------------------------------------------------
     def forward(self, scenes: Sequence[SceneTensors], views, save_for_backward: bool = False,
@@ -213,13 +240,15 @@ class Renderer:
         the call only enqueues kernels and may be captured into a HIP graph.
         """
         if outs is None:
-            outs = self.alloc_outputs(views)
+            outs = self.alloc_outputs(views, scenes)
         elif len(outs) != len(views):
             raise FeError("forward: outs needs one (depth, alpha) pair per view")
         else:
-            for (d, a), (_, cam, _, _) in zip(outs, views):
-                self._check_image(d, cam, "forward: depth output")
-                self._check_image(a, cam, "forward: alpha output")
+            for o, (si, cam, _, _) in zip(outs, views):
+                self._check_image(o[0], cam, "forward: depth output")
+                self._check_image(o[1], cam, "forward: alpha output")
+                if len(o) > 2:
+                    self._check_image(o[2], cam, "forward: feature output", scenes[si].n_features)
         sc, vw = self._structs(scenes, views, outs)
         opt = self._opt(save_for_backward)
         nbytes = ctypes.c_size_t(0)
@@ -266,38 +295,50 @@ class Renderer:
 
     # -- backward -----------------------------------------------------------------------
     def backward(self, d_depth: Sequence[Optional[torch.Tensor]], d_alpha: Sequence[Optional[torch.Tensor]],
-                 grads=None, reuse_descriptors: bool = False):
-        """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> with respect to every
-        scene's (means, prec upper triangle, weights_log), from the last forward with
-        save_for_backward=True.  ``grads`` (optional) = [(d_means, d_prec, d_w), ...] buffers
-        to accumulate into; by default fresh zeroed tensors are returned."""
+                 grads=None, reuse_descriptors: bool = False,
+                 d_features: Optional[Sequence[Optional[torch.Tensor]]] = None):
+        """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> (+ <d_features_v,
+        features_v> for views that rendered features) with respect to every scene's (means,
+        prec upper triangle, weights_log[, features]), from the last forward with
+        save_for_backward=True (its feature images untouched since).  ``grads`` (optional) =
+        [(d_means, d_prec, d_w[, d_features]), ...] buffers to accumulate into; by default fresh
+        zeroed tensors are returned, with d_features for scenes that have features."""
         if self._last is None:
             raise FeError("backward: the last forward was not run with save_for_backward=True")
         scenes, sc, vw, outs, views = self._last
         if len(d_depth) != len(outs) or len(d_alpha) != len(outs):
             raise FeError("backward: need one depth and one alpha cotangent (or None) per view")
+        if d_features is not None and len(d_features) != len(outs):
+            raise FeError("backward: d_features needs one entry (or None) per view")
         cot = (FeViewCotangent * len(outs))()
         for i in range(len(outs)):
             dd, da = d_depth[i], d_alpha[i]
-            cam = views[i][1]
+            df = None if d_features is None else d_features[i]
+            si, cam = views[i][0], views[i][1]
             for t, what in ((dd, "depth"), (da, "alpha")):
                 if t is not None:
                     self._check_image(t, cam, f"backward: {what} cotangent of view {i}")
-            cot[i] = FeViewCotangent(_ptr(dd), _ptr(da))
+            if df is not None:
+                if len(outs[i]) < 3:
+                    raise FeError(f"backward: view {i} rendered no features")
+                self._check_image(df, cam, f"backward: feature cotangent of view {i}", scenes[si].n_features)
+            cot[i] = FeViewCotangent(_ptr(dd), _ptr(da), _ptr(df))
         if grads is None:
             grads = [(torch.zeros_like(s.means), torch.zeros_like(s.prec), torch.zeros_like(s.weights_log))
+                     + ((torch.zeros_like(s.features),) if s.features is not None else ())
                      for s in scenes]
         if len(grads) != len(scenes):
             raise FeError("backward: need one (d_means, d_prec, d_weights_log) triple per scene")
         for s, g in zip(scenes, grads):
-            if len(g) != 3:
-                raise FeError("backward: need one (d_means, d_prec, d_weights_log) triple per scene")
-            for t, ref in zip(g, (s.means, s.prec, s.weights_log)):
+            refs = (s.means, s.prec, s.weights_log) + ((s.features,) if len(g) == 4 else ())
+            if len(g) not in (3, 4) or (len(g) == 4 and s.features is None):
+                raise FeError("backward: need one (d_means, d_prec, d_weights_log[, d_features]) tuple per scene")
+            for t, ref in zip(g, refs):
                 if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or not t.is_contiguous() \
                         or t.device != self.device or t.shape != ref.shape:
                     raise FeError(f"backward: gradient buffers must be contiguous float32 tensors on "
-                                  f"{self.device} shaped like the scene's (means, prec, weights_log)")
-        gs = (FeSceneGrad * len(scenes))(*[FeSceneGrad(_ptr(a), _ptr(b), _ptr(c)) for a, b, c in grads])
+                                  f"{self.device} shaped like the scene's (means, prec, weights_log, features)")
+        gs = (FeSceneGrad * len(scenes))(*[FeSceneGrad(*[_ptr(x) for x in g]) for g in grads])
         opt = self._opt(True, reuse_descriptors)
         ws = self._ws
         _lib.check(self.lib.fe_backward(_stream(self.device), sc, len(scenes), vw, len(outs), ctypes.byref(opt),
@@ -340,16 +381,41 @@ def render_camera(means, prec_full, weights_log, camera: Camera, beta_2=BETA2, b
     return d, a
 
 
-def render_func_quat(means, prec_full, weights_log, intrinsics, quat, t, beta_2=BETA2, beta_3=BETA3, **kw):
+def render_features(means, prec_full, weights_log, features, camera: Camera, beta_2=BETA2, beta_3=BETA3,
+                    tile=16, batch=64, mode="culled", device="cuda"):
+    """Depth (H, W), alpha (H, W) and the feature image (H, W, C) = w @ features per pixel, w the
+    normalised depth weights render_func_rays returns for "color, DINO features, or any other
+    per-Gaussian property" (baseline/fm_render.py:96-109), without materialising w (N, P)."""
+    st = SceneTensors(means, prec_full, weights_log, device, features=features)
+    r = _renderer(tile, batch, mode, device)
+    (d, a, f), = r.forward([st], [(0, camera, beta_2, beta_3)])
+    return d, a, f
+
+
+MAX_WEIGHT_ENTRIES = 1 << 28
+
+
+def render_func_quat(means, prec_full, weights_log, intrinsics, quat, t, beta_2=BETA2, beta_3=BETA3,
+                     return_weights=False, **kw):
     """baseline/fm_render.py:113-126 with the camera rays given by pinhole intrinsics
     (H, W, fx, fy, cx, cy) instead of an explicit (H*W, 3) ray array.  Returns
-    (depth (H*W,), alpha (H*W,), None, None) like the reference (the (N, P) weight array
-    is not materialised)."""
+    (depth (H*W,), alpha (H*W,), None, w) like the reference; w, the (N, P) normalised weights,
+    is materialised only with ``return_weights`` (as the feature image of the identity
+    features, N*P <= 2^28), else None -- use render_features to contract it with
+    per-Gaussian properties instead."""
     H, W, fx, fy, cx, cy = intrinsics
     cam = Camera(R=quat_to_rot(quat), t=np.asarray(t, dtype=np.float64), H=int(H), W=int(W), fx=float(fx),
                  fy=float(fy), cx=float(cx), cy=float(cy))
-    d, a = render_camera(means, prec_full, weights_log, cam, beta_2, beta_3, **kw)
-    return d.reshape(-1), a.reshape(-1), None, None
+    if not return_weights:
+        d, a = render_camera(means, prec_full, weights_log, cam, beta_2, beta_3, **kw)
+        return d.reshape(-1), a.reshape(-1), None, None
+    n = int(np.asarray(weights_log).shape[0]) if not isinstance(weights_log, torch.Tensor) else int(weights_log.shape[0])
+    if n * cam.H * cam.W > MAX_WEIGHT_ENTRIES or n > 65536:
+        raise ValueError("render_func_quat: the (N, P) weights are too large to materialise; use render_features")
+    dev = kw.get("device", "cuda")
+    eye = torch.eye(max(n, 1), dtype=torch.float32, device=dev)[:n]
+    d, a, f = render_features(means, prec_full, weights_log, eye, cam, beta_2, beta_3, **kw)
+    return d.reshape(-1), a.reshape(-1), None, f.reshape(-1, n).t()
 
 
 def prec_from_scale_quat(scales, quats) -> np.ndarray:

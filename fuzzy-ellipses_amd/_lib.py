@@ -34,22 +34,23 @@ class FeCamera(ctypes.Structure):
 
 class FeScene(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int32), ("means", ctypes.c_void_p), ("prec", ctypes.c_void_p),
-                ("weights_log", ctypes.c_void_p)]
+                ("weights_log", ctypes.c_void_p), ("features", ctypes.c_void_p), ("n_features", ctypes.c_int32)]
 
 
 class FeView(ctypes.Structure):
     _fields_ = [("scene", ctypes.c_int32), ("cam", FeCamera), ("beta2", ctypes.c_float),
                 ("beta3", ctypes.c_float), ("depth", ctypes.c_void_p), ("alpha", ctypes.c_void_p),
-                ("pose_device", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("n_rays", ctypes.c_int64)]
+                ("pose_device", ctypes.c_void_p), ("rays", ctypes.c_void_p), ("n_rays", ctypes.c_int64),
+                ("features_out", ctypes.c_void_p)]
 
 
 class FeSceneGrad(ctypes.Structure):
     _fields_ = [("d_means", ctypes.c_void_p), ("d_prec", ctypes.c_void_p),
-                ("d_weights_log", ctypes.c_void_p)]
+                ("d_weights_log", ctypes.c_void_p), ("d_features", ctypes.c_void_p)]
 
 
 class FeViewCotangent(ctypes.Structure):
-    _fields_ = [("d_depth", ctypes.c_void_p), ("d_alpha", ctypes.c_void_p)]
+    _fields_ = [("d_depth", ctypes.c_void_p), ("d_alpha", ctypes.c_void_p), ("d_features", ctypes.c_void_p)]
 
 
 class FeOptions(ctypes.Structure):

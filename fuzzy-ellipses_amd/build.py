@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB_DIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIB_DIR, "libfe_amd.so")
-SOURCES = ["fe_forward.hip", "fe_backward.hip", "fe_fit.hip", "fe_gmm.hip", "fe_rays.hip", "fe_xla.hip", "fe_capi.cpp"]
+SOURCES = ["fe_forward.hip", "fe_backward.hip", "fe_fit.hip", "fe_gmm.hip", "fe_rays.hip", "fe_xla.hip", "fe_features.hip", "fe_capi.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -35,6 +35,8 @@
 //   dq = (dB_0 - 2 n (n.dB_0))/|q|^2 - sum_i (e_i (n.dB_i) + n (e_i.dB_i))/|q|^2 - q sum g_d2 W r,
 // with n = q/|q|, once per (tile, Gaussian).
 #include "fe_common.h"
+#include "fe_grad.h"
+#include "fe_launch.h"
 
 #include <string.h>
 
@@ -192,9 +194,8 @@ struct BwdArgs {
   const int32_t* tord;  // dispatch order (block b takes tile tord[b]), or null
 };
 
-// Sums per record.  Planes, not records of 11 floats: device-scope float atomics to one
-// cache line serialise, and a record's 11 sums are added together by every tile.
-constexpr int kGsumPlanes = 11;
+// Sums per record (kGsumPlanes = 11).  Planes, not records of 11 floats: device-scope float
+// atomics to one cache line serialise, and a record's 11 sums are added together by every tile.
 
 // Per-pixel terms of the backward (upstream gradients through the saved aggregates).
 struct PixTerms {
@@ -562,7 +563,6 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   const float kd = -0.5f * kLog2e;
   const bool safe = (jflags & kFlagDegenerate) != 0;   // the view has degenerate records
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
-  const float ifx = 1.0f / jb.fx, ify = 1.0f / jb.fy;
 
   for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 256) {
     const int64_t kk = c0 + lane;
@@ -583,38 +583,8 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
       bwd_pixels<false>(pxa, pxb, pxc, npx, r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w, r2.x, r2.y, r2.z, r2.w,
                         r3.y, b3, b2, nb2l, kd, Gq0, Gq1, Gq2, Ga0, Ga1, Ga2, Gb0, Gb1, Gb2, Gg0, Gg1, Gg2, Gw);
     }
-    // chain rule for this (tile, Gaussian) pair, straight into the scene gradients:
-    //   dM = [G_a/fx, G_b/fy, G_g],  dA = dM R + G_q p^T (lower part),  dmu = A^T G_q,
-    //   d(prec)[k][i] += dA[i][k] for k <= i,  dw += G_w
-    const int64_t sg = gauss_of(jb, gid);   // the record's Gaussian in the scene arrays
-    const float* Pp = jb.prec + 9 * sg;
-    const float* mu = jb.means + 3 * sg;
-    const float u00 = Pp[0], u01 = Pp[1], u02 = Pp[2], u11 = Pp[4], u12 = Pp[5], u22 = Pp[8];
-    const float pv[3] = {mu[0] - jb.o[0], mu[1] - jb.o[1], mu[2] - jb.o[2]};
-    const float dm[3][3] = {{Ga0 * ifx, Gb0 * ify, Gg0}, {Ga1 * ifx, Gb1 * ify, Gg1}, {Ga2 * ifx, Gb2 * ify, Gg2}};
-    const float gq[3] = {Gq0, Gq1, Gq2};
-    float dA[3][3];
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int k = 0; k <= i; ++k)
-        dA[i][k] = fmaf(dm[i][0], jb.R[k], fmaf(dm[i][1], jb.R[3 + k], fmaf(dm[i][2], jb.R[6 + k], gq[i] * pv[k])));
-    if (bj.dmeans) {
-      float* dmu = bj.dmeans + 3 * sg;
-      atomicAdd(dmu + 0, fmaf(u00, Gq0, fmaf(u01, Gq1, u02 * Gq2)));
-      atomicAdd(dmu + 1, fmaf(u11, Gq1, u12 * Gq2));
-      atomicAdd(dmu + 2, u22 * Gq2);
-    }
-    if (bj.dprec) {
-      float* dp = bj.dprec + 9 * sg;
-      atomicAdd(dp + 0, dA[0][0]);
-      atomicAdd(dp + 1, dA[1][0]);
-      atomicAdd(dp + 2, dA[2][0]);
-      atomicAdd(dp + 4, dA[1][1]);
-      atomicAdd(dp + 5, dA[2][1]);
-      atomicAdd(dp + 8, dA[2][2]);
-    }
-    if (bj.dw) atomicAdd(bj.dw + sg, Gw);
+    // chain rule for this (tile, Gaussian) pair, straight into the scene gradients
+    direct_chain(jb, bj, gauss_of(jb, gid), Gq0, Gq1, Gq2, Ga0, Ga1, Ga2, Gb0, Gb1, Gb2, Gg0, Gg1, Gg2, Gw);
   }
 }
 
@@ -665,7 +635,7 @@ hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Jo
                             int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                             const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
                             const float* st, int64_t total_pix, const int32_t* job_flags, float* gsum,
-                            int64_t total_rec, int dense, const int32_t* tord) {
+                            int64_t total_rec, int dense, const int32_t* tord, int feat_chunks) {
   const int64_t n4 = (total_rec * kGsumPlanes + 3) / 4;   // the region is padded to 256 B
   if (n4 > 0)
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((n4 + 255) / 256, 4096)), dim3(256), 0, s,
@@ -679,6 +649,11 @@ hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Jo
     case 32: e = bwd_t<32>(s, total_tiles, a); break;
   }
   if (e != hipSuccess) return e;
+  if (feat_chunks > 0) {
+    e = launch_feat(s, true, ts, total_tiles, feat_chunks, jobs, bjobs, n_jobs, rec, rec2, off1, off2, list, st,
+                    total_pix, job_flags, gsum, total_rec, dense);
+    if (e != hipSuccess) return e;
+  }
   if (total_rec > 0)
     hipLaunchKernelGGL(k_bwd_chain, dim3((unsigned)((total_rec + 255) / 256)), dim3(256), 0, s, jobs,
                        reinterpret_cast<const BwdJob*>(bjobs), n_jobs, job_flags, gsum, total_rec);

@@ -46,6 +46,7 @@ struct Layout {
   int max_n = 0;
   int has_rays = 0;
   int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0, reuse = 0;
+  int feat_chunks = 0;   // blocks per tile of the feature kernel (0: no view wants features)
   int64_t total_stiles = 0;
   int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_rec2 = 0, o_tnp = 0,
@@ -125,6 +126,12 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       set_msg("fe_plan: image or scene too large");
       return FE_ERR_INVALID_ARGUMENT;
     }
+    if (sc.n_features < 0 || sc.n_features > (1 << 16) || (sc.n_features > 0 && sc.n > 0 && !sc.features)) {
+      set_msg("fe_plan: a scene's features need 0 <= n_features <= 65536 and a feature array");
+      return FE_ERR_INVALID_ARGUMENT;
+    }
+    const bool want_feat = sc.n_features > 0 && sc.n > 0 && vw.features_out != nullptr;
+    if (want_feat) L->feat_chunks = std::max(L->feat_chunks, feat_chunks(sc.n_features));
     // a ray list is laid out as an image one tile wide: tile t holds rays [t*ts*ts, (t+1)*ts*ts)
     const int64_t tt = (int64_t)o.tile * o.tile;
     const int Wv = ray_view ? o.tile : c.width;
@@ -153,6 +160,11 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
         jb.fx = c.fx; jb.fy = c.fy; jb.cx = c.cx; jb.cy = c.cy;
       }
       jb.npix = npix;
+      if (want_feat) {
+        jb.feat = sc.features;
+        jb.feat_out = vw.features_out;
+        jb.n_feat = sc.n_features;
+      }
       jb.beta2 = vw.beta2; jb.beta3 = vw.beta3;
       const double ln_n = log((double)std::max(sc.n, 1));
       // alpha: total dropped density <= n * exp(-c_alpha) = alpha_eps
@@ -317,6 +329,7 @@ static hipError_t timed(hipStream_t s, int k, F launch, bool may_time = true) {
 struct ArgBlob {
   unsigned char b[1024];
 };
+static_assert(256 + ((sizeof(Job) + 255) / 256) * 256 <= sizeof(ArgBlob), "one view's header and job fit the blob");
 __global__ void k_put_blob(ArgBlob blob, int bytes, unsigned char* dst) {
   for (int i = threadIdx.x; i < bytes; i += blockDim.x) dst[i] = blob.b[i];
 }
@@ -402,6 +415,9 @@ static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_s
       return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, rec2, nullptr, nullptr, nullptr,
                           nullptr, state, L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
     }, may_time));
+    if (L.feat_chunks > 0)
+      FE_HIP(launch_feat(s, false, L.ts, L.total_tiles, L.feat_chunks, dj, nullptr, L.n_jobs, rec, rec2, nullptr,
+                         nullptr, nullptr, state, L.total_pix, flags, nullptr, L.total_rec, 1));
     return FE_OK;
   }
   unsigned long long* mask = at<unsigned long long>(ws, L.o_mask);
@@ -444,6 +460,9 @@ static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_s
     return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
                         L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
   }, may_time));
+  if (L.feat_chunks > 0)
+    FE_HIP(launch_feat(s, false, L.ts, L.total_tiles, L.feat_chunks, dj, nullptr, L.n_jobs, rec, rec2, off1, off2,
+                       list, state, L.total_pix, flags, nullptr, L.total_rec, 0));
   return FE_OK;
 }
 
@@ -472,13 +491,19 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
     return FE_ERR_WORKSPACE;
   }
   std::vector<BwdJob> bj(n_views);
+  int feat_chunks_bwd = 0;
   for (int v = 0; v < n_views; ++v) {
     const fe_scene_grad& g = grads[views[v].scene];
+    const fe_scene& sc = scenes[views[v].scene];
     bj[v].gD = cot[v].d_depth;
     bj[v].gA = cot[v].d_alpha;
     bj[v].dmeans = g.d_means;
     bj[v].dprec = g.d_prec;
     bj[v].dw = g.d_weights_log;
+    const bool feat = sc.n_features > 0 && sc.n > 0 && views[v].features_out && cot[v].d_features;
+    bj[v].gF = feat ? cot[v].d_features : nullptr;
+    bj[v].dfeat = feat ? g.d_features : nullptr;
+    if (feat) feat_chunks_bwd = std::max(feat_chunks_bwd, feat_chunks(sc.n_features));
   }
   if (!L.reuse) {
     st = upload(s, at<char>(ws, L.o_bjobs), bj.data(), sizeof(BwdJob) * n_views);
@@ -494,7 +519,7 @@ fe_status backward(hipStream_t s, const fe_scene* scenes, int32_t n_scenes, cons
                             dense ? nullptr : at<unsigned long long>(ws, L.o_off2),
                             dense ? nullptr : at<uint32_t>(ws, L.o_list), at<float>(ws, L.o_state), L.total_pix,
                             at<int32_t>(ws, L.o_flags), at<float>(ws, L.o_gsum), L.total_rec, dense ? 1 : 0,
-                            dense ? nullptr : at<int32_t>(ws, L.o_tord) + L.total_tiles);
+                            dense ? nullptr : at<int32_t>(ws, L.o_tord) + L.total_tiles, feat_chunks_bwd);
   }));
   return FE_OK;
 }
@@ -578,7 +603,7 @@ fe_status fe_backward(void* stream, const fe_scene* scenes, int32_t n_scenes, co
                       workspace, workspace_bytes);
 }
 
-int32_t fe_version(void) { return 101; }
+int32_t fe_version(void) { return 102; }
 
 const char* fe_status_string(fe_status s) {
   switch (s) {

@@ -100,6 +100,11 @@ struct Job {
   // centres, in chunks of 1024, so a tile's list is a few runs of nearby ids.
   int32_t* order;
   int32_t* rank;      // inverse of order: Gaussian g is record rank[g]
+  // Optional per-Gaussian features (scene order, (n, n_feat) row-major) and this view's
+  // feature image (npix, n_feat): the pixel's depth-softmax-weighted feature sum.
+  const float* feat;
+  float* feat_out;
+  int32_t n_feat;
 };
 
 // Tile of a block of a one-block-per-tile launch of n blocks.  The hardware deals blocks
@@ -137,7 +142,12 @@ struct BwdJob {
   float* dmeans;
   float* dprec;
   float* dw;
+  const float* gF;     // cotangent of the feature image (npix, n_feat), or null
+  float* dfeat;        // d(loss)/d(features) (n, n_feat) of the scene, or null
 };
+
+// Blend-record backward: sums per (Gaussian, view) record, one plane each (see fe_backward.hip).
+constexpr int kGsumPlanes = 11;
 
 // Device counters (zeroed by k_init each forward).  The per-block evaluation counts go to
 // one of kCountSlots slots, each on its own 128-B line, picked by block index: a single

@@ -45,11 +45,21 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
                         float* tile_lmin, float* tile_seed, const float* tile_ubm, Counters* cnt,
                         const int32_t* job_flags, int save, const int32_t* tord);
 
+// With feat_chunks > 0 the feature backward (k_feat_bwd) runs between the blend and the
+// per-record chain rule.
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
                             int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                             const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
                             const float* st, int64_t total_pix, const int32_t* job_flags, float* gsum,
-                            int64_t total_rec, int dense, const int32_t* tord);
+                            int64_t total_rec, int dense, const int32_t* tord, int feat_chunks);
+// Blocks per tile of the feature kernels for n_feat channels.
+int feat_chunks(int n_feat);
+// Feature image (forward) or its backward over every tile, `chunks` blocks per tile.
+hipError_t launch_feat(hipStream_t s, bool backward, int ts, int64_t total_tiles, int chunks, const Job* jobs,
+                       const void* bjobs, int n_jobs, const float* rec, const float* rec2,
+                       const unsigned long long* off1, const unsigned long long* off2, const uint32_t* list,
+                       const float* st, int64_t total_pix, const int32_t* job_flags, float* gsum, int64_t total_rec,
+                       int dense);
 size_t bwd_gsum_floats();
 
 hipError_t launch_sil_loss(hipStream_t s, const fe_sil_view* views, int n_views, float* cnt_part,

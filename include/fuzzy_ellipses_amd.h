@@ -60,6 +60,12 @@ typedef struct fe_scene {
   const float* means;        /* (N,3)   */
   const float* prec;         /* (N,3,3) upper triangle read */
   const float* weights_log;  /* (N,)    */
+  /* Optional per-Gaussian features (N, n_features) row-major (colour, descriptors, ...), or
+   * NULL / 0.  Views of this scene with a feature image get, per pixel, the features weighted
+   * by render_func_rays' normalised depth weights w (baseline/fm_render.py:96-109):
+   * F = sum_j w_j f_j -- the reference's w contracted with f, without materialising w (N, P). */
+  const float* features;
+  int32_t n_features;
 } fe_scene;
 
 /* One view of a scene: camera, beta hyper-parameters and output images. */
@@ -82,6 +88,9 @@ typedef struct fe_view {
    * position cull best; any order gives the same images to within the culling budgets. */
   const float* rays;
   int64_t n_rays;
+  /* Optional feature image (H*W or n_rays, n_features) row-major, written by fe_forward when
+   * the view's scene has features (NULL = not wanted).  fe_backward reads it back. */
+  float* features_out;
 } fe_view;
 
 /* Gradient buffers of one scene (device, accumulated into: zero them first). */
@@ -89,12 +98,14 @@ typedef struct fe_scene_grad {
   float* d_means;        /* (N,3)   */
   float* d_prec;         /* (N,3,3) only the upper triangle is written (lower stays as given) */
   float* d_weights_log;  /* (N,)    */
+  float* d_features;     /* (N, n_features) or NULL */
 } fe_scene_grad;
 
-/* Upstream gradients of one view (device, (H,W) each; either may be NULL = zero). */
+/* Upstream gradients of one view (device, (H,W) each; any may be NULL = zero). */
 typedef struct fe_view_cotangent {
   const float* d_depth;
   const float* d_alpha;
+  const float* d_features;   /* (H*W or n_rays, n_features) cotangent of the feature image */
 } fe_view_cotangent;
 
 enum {

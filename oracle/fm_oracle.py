@@ -85,6 +85,26 @@ def combine(zs, stds, beta_2, beta_3):
     return depth, alpha
 
 
+def weights(zs, stds, beta_2, beta_3):
+    """The normalised per-(ray, Gaussian) weights w that render_func_rays returns as its fourth
+    output (baseline/fm_render.py:93-99), (P, N): (t*>0) nan_to_num(stable_exp(l)) + 1e-20 with
+    l = -b2 t* + b3 d2, normalised over the Gaussians."""
+    with np.errstate(over="ignore", invalid="ignore"):
+        sig1 = zs > 0
+        logit = -zs * beta_2 + beta_3 * stds
+        se = np.exp(logit - np.max(logit, axis=1, keepdims=True))   # util_render.py:5-9
+        wgt = sig1 * np.nan_to_num(se) + 1e-20
+        return wgt / np.sum(wgt, axis=1, keepdims=True)
+
+
+def render_weights(rays, starts, means, prec_full, weights_log, beta_2, beta_3):
+    """(P, N) weights w of P rays (fm_render.py:96-109); features f (N, C) give w @ f."""
+    rays = np.asarray(rays, dtype=np.float64)
+    starts = np.broadcast_to(np.asarray(starts, dtype=np.float64), rays.shape)
+    zs, d2 = pair_terms(rays, starts, means, prec_full, weights_log)
+    return weights(zs, d2, beta_2, beta_3)
+
+
 def render_func_rays(rays, starts, means, prec_full, weights_log, beta_2, beta_3, chunk=None):
     """Dense FP64 render of P rays -> (depth (P,), alpha (P,)).
 

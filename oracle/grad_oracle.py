@@ -11,8 +11,10 @@ import numpy as np
 import torch
 
 
-def render_rays_torch(rays, starts, means, prec_full, weights_log, beta_2, beta_3):
-    """(depth, alpha) for rays (P,3) starting at starts (P,3); float64 torch tensors."""
+def render_rays_torch(rays, starts, means, prec_full, weights_log, beta_2, beta_3, features=None):
+    """(depth, alpha) for rays (P,3) starting at starts (P,3); float64 torch tensors.  With
+    ``features`` (N, C) also the feature image w @ features (P, C), w the normalised weights
+    render_func_rays returns (fm_render.py:96-109)."""
     prc = torch.triu(prec_full).transpose(1, 2)                     # (N,3,3) lower
     p = means[None, :, :] - starts[:, None, :]                      # (P,N,3)
     projp = torch.einsum("nij,pnj->pni", prc, p)
@@ -31,6 +33,8 @@ def render_rays_torch(rays, starts, means, prec_full, weights_log, beta_2, beta_
     w = sig1 * torch.nan_to_num(se) + 1e-20
     w = w / w.sum(1, keepdim=True)
     depth = (w * torch.nan_to_num(res)).sum(1)
+    if features is not None:
+        return depth, alpha, w @ features
     return depth, alpha
 
 
@@ -41,6 +45,30 @@ def pinhole_rays(cam, pixels=None):
     y = (idx // W).astype(np.float64)
     rc = np.stack([(x - cam.cx) / cam.fx, (y - cam.cy) / cam.fy, np.ones_like(x)], 1)
     return rc @ np.asarray(cam.R, dtype=np.float64)
+
+
+def feature_grads(scene, features, cams, d_depth, d_alpha, d_features, chunk=2048):
+    """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> + <d_features_v, F_v>
+    (F = w @ features) with respect to (means, prec upper triangle, weights_log, features),
+    float64 autograd."""
+    means = torch.tensor(scene.means, dtype=torch.float64, requires_grad=True)
+    prec = torch.tensor(scene.prec, dtype=torch.float64, requires_grad=True)
+    w = torch.tensor(scene.weights_log, dtype=torch.float64, requires_grad=True)
+    f = torch.tensor(np.asarray(features, dtype=np.float64), requires_grad=True)
+    for v, cam in enumerate(cams):
+        rays_all = pinhole_rays(cam)
+        gd = np.asarray(d_depth[v], dtype=np.float64).reshape(-1)
+        ga = np.asarray(d_alpha[v], dtype=np.float64).reshape(-1)
+        gf = np.asarray(d_features[v], dtype=np.float64).reshape(rays_all.shape[0], -1)
+        for s in range(0, rays_all.shape[0], chunk):
+            e = min(rays_all.shape[0], s + chunk)
+            rays = torch.tensor(rays_all[s:e])
+            starts = torch.tensor(np.repeat(np.asarray(cam.t, dtype=np.float64)[None], e - s, 0))
+            d, a, F = render_rays_torch(rays, starts, means, prec, w, scene.beta2, scene.beta3, features=f)
+            loss = (d * torch.tensor(gd[s:e])).sum() + (a * torch.tensor(ga[s:e])).sum() + \
+                (F * torch.tensor(gf[s:e])).sum()
+            loss.backward()
+    return means.grad.numpy(), np.triu(prec.grad.numpy()), w.grad.numpy(), f.grad.numpy()
 
 
 def grads(scene, cams, d_depth, d_alpha, chunk=2048, pixels=None):

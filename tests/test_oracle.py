@@ -227,3 +227,29 @@ def test_golden_fixture(scenes_mod):
     d, a = ref.render_f64(scene, scene.cameras[0])
     np.testing.assert_allclose(d, g["depth"], rtol=1e-12)
     np.testing.assert_allclose(a, g["alpha"], rtol=1e-12, atol=1e-15)
+
+
+def test_weights_output_closed_forms():
+    """render_func_rays' fourth output w (fm_render.py:96-109): columns sum to one, depth is
+    w @ nan_to_num(t*), two unmasked Gaussians give the closed-form softmax of their logits,
+    and an all-masked ray gives the uniform 1e-20 floor (1/N each)."""
+    rng = np.random.default_rng(5)
+    n = 6
+    means = rng.uniform(-0.3, 0.3, size=(n, 3)) + np.array([0.0, 0.0, 3.0])
+    U = np.tile(np.eye(3) * 4.0, (n, 1, 1))
+    wl = rng.normal(size=n)
+    rays = np.concatenate([rng.uniform(-0.1, 0.1, size=(7, 2)), np.ones((7, 1))], 1)
+    starts = np.zeros((7, 3))
+    w = fo.render_weights(rays, starts, means, U, wl, 21.4, 2.66)
+    np.testing.assert_allclose(w.sum(1), 1.0, rtol=1e-14)
+    zs, d2 = fo.pair_terms(rays, starts, means, U, wl)
+    d, _ = fo.render_func_rays(rays, starts, means, U, wl, 21.4, 2.66)
+    np.testing.assert_allclose((w * np.nan_to_num(zs)).sum(1), d, rtol=1e-12)
+    # two Gaussians in front: softmax of l = b3 d2 - b2 t (the floor is 1e-20 of it)
+    two = fo.render_weights(rays[:1], starts[:1], means[:2], U[:2], wl[:2], 21.4, 2.66)
+    l = 2.66 * d2[0, :2] - 21.4 * zs[0, :2]
+    e = np.exp(l - l.max())
+    np.testing.assert_allclose(two[0], e / e.sum(), rtol=1e-12)
+    # every Gaussian behind the ray start: only the floor, uniform
+    back = fo.render_weights(-rays[:1], starts[:1], means, U, wl, 21.4, 2.66)
+    np.testing.assert_allclose(back[0], np.full(n, 1.0 / n), rtol=1e-12)
