@@ -61,12 +61,17 @@ class Scheduler:
     """
 
     def __init__(self, n_slots: int, max_batch: int, free_bytes: int, launch, ready, bytes_of,
-                 clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep):
+                 clock: Callable[[], float] = time.perf_counter, sleep: Callable[[float], None] = time.sleep,
+                 held: Optional[Callable[[], int]] = None, retire: Optional[Callable[[int], None]] = None):
         self.n_slots = max(1, int(n_slots))
         self.max_batch = max(1, int(max_batch))
         self.free_bytes = int(free_bytes)
         self.launch, self.ready, self.bytes_of = launch, ready, bytes_of
         self.clock, self.sleep = clock, sleep
+        # memory held outside the in-flight requests (e.g. idle slots' workspaces), and a
+        # callback told the slot whose batch retired
+        self.held = held if held is not None else (lambda: 0)
+        self.retire = retire if retire is not None else (lambda k: None)
         self.peak_in_flight = 0
         self.batch_sizes: List[int] = []
 
@@ -92,14 +97,16 @@ class Scheduler:
                     done += sl[1]
                     in_flight -= sl[2]
                     slots[k] = None
+                    self.retire(k)
                     progressed = True
             idle = [k for k, sl in enumerate(slots) if sl is None]
             if queue and idle:
                 batch: List[Request] = []
                 nbytes = 0
+                held = self.held()
                 while queue and len(batch) < self.max_batch:
                     b = self.bytes_of(queue[0])
-                    if in_flight + nbytes + b > self.free_bytes:
+                    if in_flight + held + nbytes + b > self.free_bytes:
                         break
                     batch.append(queue.popleft())
                     nbytes += b
@@ -110,7 +117,7 @@ class Scheduler:
                         r.t_launch = t
                     slots[k] = (self.launch(k, batch), batch, nbytes)
                     in_flight += nbytes
-                    self.peak_in_flight = max(self.peak_in_flight, in_flight)
+                    self.peak_in_flight = max(self.peak_in_flight, in_flight + self.held())
                     self.batch_sizes.append(len(batch))
                     progressed = True
                 elif len(idle) == self.n_slots:
@@ -150,22 +157,43 @@ class Server:
         self.max_batch = int(max_batch)
         self.tile = tile
         self.slots = [_Slot(self.device, tile, batch) for _ in range(max(1, slots))]
+        self._slot_ws = [0] * len(self.slots)    # workspace shares of the batch in flight per slot
+        self._ws_cache: Dict[tuple, int] = {}
         self.clock = clock
 
+    def workspace_share(self, req: Request) -> int:
+        """The workspace the planner (fe_plan) sizes for this request's view alone (with the
+        backward's planes for a forward+backward request), cached per (N, H, W, backward)."""
+        key = (self.scenes[req.scene].n, req.camera.H, req.camera.W, bool(req.backward))
+        b = self._ws_cache.get(key)
+        if b is None:
+            b = self.slots[0].fwd.workspace_bytes([self.scenes[req.scene]], [(0, req.camera, req.beta2, req.beta3)],
+                                                  save_for_backward=req.backward)
+            self._ws_cache[key] = b
+        return b
+
     def request_bytes(self, req: Request) -> int:
-        """Bytes a request holds while in flight: its share of the workspace (tile lists sized
-        for the worst case, records, masks, per-pixel state), outputs and gradient buffers."""
+        """Bytes a request holds while in flight: its workspace share (as planned by fe_plan:
+        records, masks, worst-case tile lists, per-pixel state, backward planes), outputs and,
+        for a backward, its gradient buffers (the cotangents are the caller's)."""
         n = self.scenes[req.scene].n
-        cam = req.camera
-        tiles = math.ceil(cam.W / self.tile) * math.ceil(cam.H / self.tile)
-        b = 4 * n * tiles + 64 * n + 24 * cam.H * cam.W + 16 * tiles * math.ceil(n / 64) + 32 * tiles
-        b += 8 * cam.H * cam.W                      # depth + alpha
+        b = self.workspace_share(req) + 8 * req.camera.H * req.camera.W       # + depth, alpha
         if req.backward:
-            b += 52 * n + 8 * cam.H * cam.W         # gradients + cotangents
+            b += 52 * n                                                          # d_means, d_prec, d_w
         return b + 4096
+
+    def held_bytes(self) -> int:
+        """Workspace the slots' renderers keep (at their high-water mark) beyond the shares of
+        the requests now in flight on them."""
+        tot = 0
+        for k, slot in enumerate(self.slots):
+            cap = sum(r._ws.numel() if r._ws is not None else 0 for r in (slot.fwd, slot.fb))
+            tot += max(0, cap - self._slot_ws[k])
+        return tot
 
     def _launch(self, k: int, batch: List[Request]):
         slot = self.slots[k]
+        self._slot_ws[k] = sum(self.workspace_share(r) for r in batch)
         with torch.cuda.stream(slot.stream):
             groups = ([r for r in batch if not r.backward], [r for r in batch if r.backward])
             for g, rnd in zip(groups, (slot.fwd, slot.fb)):
@@ -185,10 +213,14 @@ class Server:
             ev.record(slot.stream)
         return ev
 
+    def _retire(self, k: int) -> None:
+        self._slot_ws[k] = 0
+
     def run(self, requests: Sequence[Request]) -> Dict[str, float]:
         torch.cuda.synchronize(self.device)      # inputs (cotangents) are ready
         sch = Scheduler(len(self.slots), self.max_batch, self.budget - self.scene_bytes, self._launch,
-                        lambda ev: ev.query(), self.request_bytes, clock=self.clock)
+                        lambda ev: ev.query(), self.request_bytes, clock=self.clock, held=self.held_bytes,
+                        retire=self._retire)
         t0 = self.clock()
         done = sch.run(requests)
         total = self.clock() - t0

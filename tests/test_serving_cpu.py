@@ -93,3 +93,42 @@ def test_plain_path_is_one_request_per_launch(fea):
     dev = FakeDevice(clock)
     done = _sched(fea, clock, dev, slots=1, max_batch=1).run(_reqs(fea, 10, rate=1000))
     assert len(done) == 10 and all(len(b) == 1 for _, b, _ in dev.launched)
+
+
+def test_held_memory_counts_against_the_budget(fea):
+    """Memory held outside the in-flight requests (idle slots' workspaces) shrinks what a batch
+    may take; the retire callback names each slot whose batch finished."""
+    clock = FakeClock()
+    dev = FakeDevice(clock)
+    from fuzzy_ellipses_amd.serve import Scheduler
+    retired = []
+    sch = Scheduler(2, 64, 600, dev.launch, dev.ready, lambda r: 100, clock=clock, sleep=clock.sleep,
+                    held=lambda: 300, retire=retired.append)
+    done = sch.run(_reqs(fea, 12))
+    assert len(done) == 12
+    assert max(sch.batch_sizes) <= 3 and sch.peak_in_flight <= 600
+    assert len(retired) == len(sch.batch_sizes) and set(retired) <= {0, 1}
+
+
+def test_planner_sizes_what_requests_are_charged(fea):
+    """Server.request_bytes charges fe_plan's workspace for the request's view (planning is host
+    arithmetic: no GPU needed); that workspace holds at least the worst-case tile lists, 4 bytes
+    per (tile, Gaussian), with or without the backward's planes."""
+    import ctypes
+    from fuzzy_ellipses_amd import _lib
+    import fuzzy_ellipses_amd.scenes as sc
+    lib = _lib.load()
+    cam = sc.make_camera([0, 0, 3.0], 240, 320, 300.0)
+    for n, bwd in ((500, False), (4096, True)):
+        scn = (_lib.FeScene * 1)(_lib.FeScene(n, None, None, None))
+        c = _lib.FeCamera()
+        R = np.asarray(cam.R, np.float32).ravel()
+        for i in range(9):
+            c.R[i] = float(R[i])
+        c.fx, c.fy, c.cx, c.cy, c.height, c.width = cam.fx, cam.fy, cam.cx, cam.cy, cam.H, cam.W
+        vw = (_lib.FeView * 1)(_lib.FeView(0, c, 21.4, 2.66, None, None))
+        nbytes = ctypes.c_size_t(0)
+        assert lib.fe_plan(scn, 1, vw, 1, ctypes.byref(_lib.FeOptions(16, 64, 0, int(bwd), 0)),
+                           ctypes.byref(nbytes)) == 0
+        # the planner's worst-case lists alone are 4 bytes per (tile, Gaussian)
+        assert nbytes.value >= 4 * n * 15 * 20
