@@ -994,27 +994,48 @@ struct BlendArgs {
 // Pixels per thread of the blend: two for 16- and 32-pixel tiles (each record read from LDS
 // serves two pixels; with one, three 16-byte LDS broadcasts per pair bound the kernel), one
 // for 8-pixel tiles (whose 64 pixels fill one wave).
-template <int TS>
+// A tile's list is split KS ways inside the block: each round of GB staged records is shared
+// out in KS contiguous slices, one per group of NT threads, and the groups' per-pixel states
+// are merged at the end (max, then S and Z rescaled to it; E and Zall added).  A lone tile then
+// runs on KS times the waves: the blend of one small view is bound by its longest tile's
+// latency, not by throughput (a 128x128 view with N=4096 took 0.76 ms with one group, the same
+// as 16 such views).  The split is fixed per tile size, so a tile's result does not depend on
+// the batch around it.
+template <int TS, int PHASE>
 struct BlendShape {
   static constexpr int PPT = (TS >= 16) ? 2 : 1;
-  static constexpr int NT = TS * TS / PPT;   // threads per block
+  static constexpr int NT = TS * TS / PPT;   // threads per list split (one or two pixels each)
+  // list splits per block: blocks of 4 waves, as many as 7 waves per SIMD hold exactly (28
+  // per CU; 8-wave blocks fit only 3 times, and the phase-1 blend of the 16-view shell batch
+  // took 1.10 ms with 4 splits of 2 waves against 0.89 unsplit).  The phase-2 lists are short
+  // (its blend is bound by loading and storing the pixel state), so it keeps one group, as do
+  // 32-pixel tiles (8 waves already).
+  static constexpr int KS = (PHASE == 2 || NT >= 256) ? 1 : 256 / NT;
+  static constexpr int NB = NT * KS;         // threads per block
 };
 
 // The phase-2 blend of 16-pixel tiles is held to 128 VGPRs (4 waves per SIMD instead of 3):
 // its tiles are short and its time is mostly loading and storing the per-pixel state, which
 // more resident waves hide better (0.112 -> 0.101 ms).
 template <int TS, int GB, int PHASE>
-__global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 1) void k_blend(BlendArgs A) {
-  constexpr int PPT = BlendShape<TS>::PPT;
-  constexpr int NT = BlendShape<TS>::NT;
+__global__ __launch_bounds__((BlendShape<TS, PHASE>::NB), ((TS == 16) ? (PHASE == 1 ? 7 : (PHASE == 2 ? 4 : 1)) : 1))
+void k_blend(BlendArgs A) {
+  constexpr int PPT = BlendShape<TS, PHASE>::PPT;
+  constexpr int NT = BlendShape<TS, PHASE>::NT;
+  constexpr int KS = BlendShape<TS, PHASE>::KS;
+  constexpr int NB = BlendShape<TS, PHASE>::NB;
   // records per online-max update: 2 pairs per thread in the first pass, 4 in the others.
   // Fewer pairs in flight need fewer registers (70 and 88 VGPRs for 16-pixel tiles instead of
   // 112 and 128), and the extra resident waves hide the LDS and transcendental latencies: with
   // 8 pairs the first pass took 0.88 ms, with 2 it takes 0.82-0.83 ms, and the second pass
   // 0.093 -> 0.087 ms with 4.
   constexpr int SUB = (PHASE == 1) ? 2 / PPT : 4 / PPT;
+  static_assert(GB % (KS * SUB) == 0, "a round's slices fit the LDS batch");
   __shared__ float4 lds[GB * 4];
-  __shared__ float sh_red[NT / 64 + 1];
+  __shared__ float sh_red[NB / 64 + 1];
+  __shared__ float mrg[(KS > 1 ? KS - 1 : 1) * 6][NT * PPT];   // splits 1..KS-1: M S Z E Zall nan per pixel
+  const int split = __builtin_amdgcn_readfirstlane((int)threadIdx.x / NT);   // list split (wave-uniform)
+  const int ti = (int)threadIdx.x - split * NT;    // its pixel thread within the split
 
   // one block per tile
   const int64_t gt = A.tord ? (int64_t)A.tord[blockIdx.x] : xcd_tile(blockIdx.x, gridDim.x);
@@ -1022,19 +1043,19 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
   const Job& jb = A.jobs[ji];
   const int tile = (int)(gt - jb.tile_base);
   const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
-  // pixel q of this thread: tile pixel index PPT*threadIdx.x + q, so a thread's pixels are
-  // neighbours in one row (same y: the blend records' y-parts are shared between them)
+  // pixel q of this thread: tile pixel index PPT*ti + q, so a thread's pixels are neighbours
+  // in one row (same y: the blend records' y-parts are shared between them)
   bool inimg[PPT];
-  int64_t pix[PPT];
+  int32_t lpx[PPT];   // the pixel's index within the view (32 bits: fewer registers live in the loop)
   float xf[PPT], yf[PPT];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
-    const int i = PPT * threadIdx.x + q;
+    const int i = PPT * ti + q;
     const int px = tx * TS + (i % TS);
     const int py = ty * TS + (i / TS);
     const int64_t lp = (int64_t)py * jb.W + px;
     inimg[q] = (px < jb.W) && (py < jb.H) && (lp < jb.npix);
-    pix[q] = jb.pix_base + (inimg[q] ? lp : 0);
+    lpx[q] = inimg[q] ? (int32_t)lp : 0;
     if (jb.ray_xy) {
       xf[q] = inimg[q] ? jb.ray_xy[2 * lp] : 0.0f;
       yf[q] = inimg[q] ? jb.ray_xy[2 * lp + 1] : 0.0f;
@@ -1056,20 +1077,20 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
   bool nanf[PPT];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
-    if (PHASE == 2) {
-      M[q] = A.st[pix[q]];
-      S[q] = A.st[A.total_pix + pix[q]];
-      Z[q] = A.st[2 * A.total_pix + pix[q]];
-      E[q] = A.st[3 * A.total_pix + pix[q]];
-      Zall[q] = A.st[4 * A.total_pix + pix[q]];
+    if (PHASE == 2 && split == 0) {   // the other splits start empty and are merged into it
+      M[q] = A.st[jb.pix_base + lpx[q]];
+      S[q] = A.st[A.total_pix + jb.pix_base + lpx[q]];
+      Z[q] = A.st[2 * A.total_pix + jb.pix_base + lpx[q]];
+      E[q] = A.st[3 * A.total_pix + jb.pix_base + lpx[q]];
+      Zall[q] = A.st[4 * A.total_pix + jb.pix_base + lpx[q]];
       // a NaN logit seen by the phase-1 pass (kept in the sixth state plane between the passes)
       // (only the NaN-safe blend of a view with degenerate records sets it)
-      nanf[q] = safe && A.st[5 * A.total_pix + pix[q]] != 0.0f;
+      nanf[q] = safe && A.st[5 * A.total_pix + jb.pix_base + lpx[q]] != 0.0f;
     } else {
       M[q] = -3.402823466e38f; S[q] = 0.f; Z[q] = 0.f; E[q] = 0.f; Zall[q] = 0.f;
       nanf[q] = false;
     }
-    Mt[q] = (r2 && PHASE == 2) ? M[q] / b3 : -3.402823466e38f;
+    Mt[q] = (r2 && PHASE == 2 && split == 0) ? M[q] / b3 : -3.402823466e38f;
   }
 
   int64_t start, end;
@@ -1081,10 +1102,11 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
 
   for (int64_t base = start; base < end; base += GB) {
     const int nb = (int)min<int64_t>(GB, end - base);
-    const int nbr = (nb + SUB - 1) / SUB * SUB;
+    const int sq = (nb + KS * SUB - 1) / (KS * SUB) * SUB;   // records per split this round
+    const int nbr = sq * KS;
     if (r2) {
       // 3 float4 per record; null record: X = 0, Y1 = 1, wL = -inf -> contributes nothing
-      for (int i = threadIdx.x; i < nbr * 3; i += NT) {
+      for (int i = threadIdx.x; i < nbr * 3; i += NB) {
         const int k = i / 3, part = i - 3 * k;
         float4 v;
         if (k < nb) {
@@ -1098,17 +1120,18 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
         lds[i] = v;
       }
       __syncthreads();
+      const float4* sl = lds + split * sq * 3;   // this split's slice
       if (PPT > 1 && !jb.ray_xy) {
-        if (masked) blend_batch_rec2<SUB, true, PPT, true>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-        else blend_batch_rec2<SUB, false, PPT, true>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        if (masked) blend_batch_rec2<SUB, true, PPT, true>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        else blend_batch_rec2<SUB, false, PPT, true>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
       } else {
-        if (masked) blend_batch_rec2<SUB, true, PPT, false>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-        else blend_batch_rec2<SUB, false, PPT, false>(lds, nbr, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        if (masked) blend_batch_rec2<SUB, true, PPT, false>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        else blend_batch_rec2<SUB, false, PPT, false>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
       }
       __syncthreads();
       continue;
     }
-    for (int i = threadIdx.x; i < nbr * 4; i += NT) {
+    for (int i = threadIdx.x; i < nbr * 4; i += NB) {
       const int k = i >> 2, part = i & 3;
       float4 v;
       if (k < nb) {
@@ -1125,10 +1148,52 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
-      if (safe) blend_batch<SUB, true>(lds, nbr, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
-      else blend_batch<SUB, false>(lds, nbr, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
+      const float4* sl = lds + split * sq * 4;
+      if (safe) blend_batch<SUB, true>(sl, sq, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
+      else blend_batch<SUB, false>(sl, sq, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
     }
     __syncthreads();
+  }
+
+  // merge the splits' states into split 0: the maximum, S and Z rescaled to it, E, Zall added
+  if (KS > 1 && split > 0) {
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const int pi = ti * PPT + q;
+      float* m = &mrg[(split - 1) * 6][0];
+      m[pi] = M[q];
+      m[NT * PPT + pi] = S[q];
+      m[2 * NT * PPT + pi] = Z[q];
+      m[3 * NT * PPT + pi] = E[q];
+      m[4 * NT * PPT + pi] = Zall[q];
+      m[5 * NT * PPT + pi] = nanf[q] ? 1.0f : 0.0f;
+    }
+  }
+  if (KS > 1) __syncthreads();
+  if (KS > 1 && split == 0) {
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+      const int pi = ti * PPT + q;
+      float Mm = M[q];
+#pragma unroll
+      for (int k = 1; k < KS; ++k) Mm = fmaxf(Mm, mrg[(k - 1) * 6][pi]);
+      // equal maxima (infinite ones included) keep their sums as they are
+      const float s0 = (M[q] == Mm) ? 1.0f : __builtin_amdgcn_exp2f(M[q] - Mm);
+      S[q] *= s0;
+      Z[q] *= s0;
+#pragma unroll
+      for (int k = 1; k < KS; ++k) {
+        const float* m = &mrg[(k - 1) * 6][0];
+        const float Mk = m[pi];
+        const float sk = (Mk == Mm) ? 1.0f : __builtin_amdgcn_exp2f(Mk - Mm);
+        S[q] = fmaf(m[NT * PPT + pi], sk, S[q]);
+        Z[q] = fmaf(m[2 * NT * PPT + pi], sk, Z[q]);
+        E[q] += m[3 * NT * PPT + pi];
+        Zall[q] += m[4 * NT * PPT + pi];
+        nanf[q] = nanf[q] || m[5 * NT * PPT + pi] != 0.0f;
+      }
+      M[q] = Mm;
+    }
   }
 
   const unsigned long long ev = (unsigned long long)(end - start) *
@@ -1143,13 +1208,13 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
     float lm = INFINITY, gap = -INFINITY, m0 = -INFINITY;
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
-      if (inimg[q]) {
-        A.st[pix[q]] = M[q];
-        A.st[A.total_pix + pix[q]] = S[q];
-        A.st[2 * A.total_pix + pix[q]] = Z[q];
-        A.st[3 * A.total_pix + pix[q]] = E[q];
-        A.st[4 * A.total_pix + pix[q]] = Zall[q];
-        if (safe) A.st[5 * A.total_pix + pix[q]] = nanf[q] ? 1.0f : 0.0f;
+      if (split == 0 && inimg[q]) {
+        A.st[jb.pix_base + lpx[q]] = M[q];
+        A.st[A.total_pix + jb.pix_base + lpx[q]] = S[q];
+        A.st[2 * A.total_pix + jb.pix_base + lpx[q]] = Z[q];
+        A.st[3 * A.total_pix + jb.pix_base + lpx[q]] = E[q];
+        A.st[4 * A.total_pix + jb.pix_base + lpx[q]] = Zall[q];
+        if (safe) A.st[5 * A.total_pix + jb.pix_base + lpx[q]] = nanf[q] ? 1.0f : 0.0f;
         const bool pos = S[q] > 0.0f && !nanf[q];
         const float l2s = pos ? __log2f(S[q]) : 0.0f;
         lm = fminf(lm, pos ? M[q] + l2s : -INFINITY);
@@ -1158,9 +1223,9 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
         else m0 = fmaxf(m0, M[q]);
       }
     }
-    lm = block_ext<NT>(lm, sh_red);
-    gap = block_ext<NT, true>(gap, sh_red);
-    m0 = block_ext<NT, true>(m0, sh_red);
+    lm = block_ext<NB>(lm, sh_red);
+    gap = block_ext<NB, true>(gap, sh_red);
+    m0 = block_ext<NB, true>(m0, sh_red);
     if (threadIdx.x == 0) {
       A.tile_lmin[gt] = lm;
       A.tile_seed[gt] = (gap >= jb.floor_thr) ? INFINITY : m0;
@@ -1183,18 +1248,18 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
     else if (nanf[q]) D = Zall[q] / ncnt;
     else D = (Z[q] + 1e-20f * Zall[q]) / (S[q] + 1e-20f * ncnt);
     const float alpha = -expm1f(-E[q]);
-    if (inimg[q]) {
-      jb.depth[pix[q] - jb.pix_base] = D;
-      jb.alpha[pix[q] - jb.pix_base] = alpha;
+    if (split == 0 && inimg[q]) {
+      jb.depth[lpx[q]] = D;
+      jb.alpha[lpx[q]] = alpha;
       if (A.save) {
         // what the backward reads: M, S, Z, E (unchanged by a second pass with no pairs) and D
         if (PHASE == 0 || end > start) {
-          A.st[pix[q]] = M[q];
-          A.st[A.total_pix + pix[q]] = S[q];
-          A.st[2 * A.total_pix + pix[q]] = Z[q];
-          A.st[3 * A.total_pix + pix[q]] = E[q];
+          A.st[jb.pix_base + lpx[q]] = M[q];
+          A.st[A.total_pix + jb.pix_base + lpx[q]] = S[q];
+          A.st[2 * A.total_pix + jb.pix_base + lpx[q]] = Z[q];
+          A.st[3 * A.total_pix + jb.pix_base + lpx[q]] = E[q];
         }
-        A.st[5 * A.total_pix + pix[q]] = D;
+        A.st[5 * A.total_pix + jb.pix_base + lpx[q]] = D;
       }
     }
     if (PHASE == 2) {
@@ -1204,7 +1269,7 @@ __global__ __launch_bounds__(BlendShape<TS>::NT, (PHASE == 2 && TS == 16) ? 4 : 
       const float lhs = -43.185f + __log2f((float)max(jb.n, 1)) + fmaxf(0.0f, ubm - M[q]);
       // (a view with degenerate records lists every pair, and so does a tile completed
       // densely: nothing was dropped there and the floor term is exact)
-      fl = fl || (inimg[q] && !safe && ncnt < (float)jb.n &&
+      fl = fl || (split == 0 && inimg[q] && !safe && ncnt < (float)jb.n &&
                   !(lhs < ((S[q] > 0.0f) ? __log2f(S[q]) : -INFINITY)));
     }
   }
@@ -1312,10 +1377,10 @@ hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t
 
 template <int TS, int GB>
 static hipError_t launch_blend_tg(hipStream_t s, int phase, int64_t total_tiles, const BlendArgs& a) {
-  const dim3 grid((unsigned)total_tiles), block(BlendShape<TS>::NT);
-  if (phase == 0) hipLaunchKernelGGL((k_blend<TS, GB, 0>), grid, block, 0, s, a);
-  else if (phase == 1) hipLaunchKernelGGL((k_blend<TS, GB, 1>), grid, block, 0, s, a);
-  else hipLaunchKernelGGL((k_blend<TS, GB, 2>), grid, block, 0, s, a);
+  const dim3 grid((unsigned)total_tiles);
+  if (phase == 0) hipLaunchKernelGGL((k_blend<TS, GB, 0>), grid, dim3(BlendShape<TS, 0>::NB), 0, s, a);
+  else if (phase == 1) hipLaunchKernelGGL((k_blend<TS, GB, 1>), grid, dim3(BlendShape<TS, 1>::NB), 0, s, a);
+  else hipLaunchKernelGGL((k_blend<TS, GB, 2>), grid, dim3(BlendShape<TS, 2>::NB), 0, s, a);
   return hipGetLastError();
 }
 
