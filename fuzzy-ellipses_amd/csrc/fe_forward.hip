@@ -15,6 +15,9 @@
 // threshold; the phase-1 blend yields, per tile, the exact minimum over its pixels of the
 // log-sum-exp of the listed depth logits; phase 2 adds every pair whose logit upper bound is
 // within the depth margin of it and finishes the pixels.  Dense mode evaluates all pairs.
+#include <atomic>
+#include <type_traits>
+
 #include "fe_common.h"
 
 namespace fe {
@@ -994,48 +997,64 @@ struct BlendArgs {
 // Pixels per thread of the blend: two for 16- and 32-pixel tiles (each record read from LDS
 // serves two pixels; with one, three 16-byte LDS broadcasts per pair bound the kernel), one
 // for 8-pixel tiles (whose 64 pixels fill one wave).
-// A tile's list is split KS ways inside the block: each round of GB staged records is shared
-// out in KS contiguous slices, one per group of NT threads, and the groups' per-pixel states
-// are merged at the end (max, then S and Z rescaled to it; E and Zall added).  A lone tile then
-// runs on KS times the waves: the blend of one small view is bound by its longest tile's
-// latency, not by throughput (a 128x128 view with N=4096 took 0.76 ms with one group, the same
-// as 16 such views).  The split is fixed per tile size, so a tile's result does not depend on
-// the batch around it.
+// A tile's list of L records is split KS ways: virtual split v blends the contiguous range
+// [v h, min(L, (v + 1) h)) of it (h = L/KS rounded up to a multiple of SUB) from its own empty
+// state, and the KS states are merged at the end (max, then S and Z rescaled to it; E and Zall
+// added, NaN flags or-ed).  The arithmetic depends on KS alone, which is fixed per tile size and
+// phase, so a tile's result does not depend on the batch around it.  How the splits run is
+// chosen per launch (SPLIT): side by side on KS groups of NT threads, each round staging R =
+// GB/KS records of every range (a tile then runs on KS times the waves, for grids whose time is
+// set by their longest tiles' latency), or one after another on one group in rounds of GB
+// records, for a grid that fills the GPU many times over (fewer instructions per pair and
+// 2-wave blocks, which wait less at the round barriers).
 template <int TS, int PHASE>
 struct BlendShape {
   static constexpr int PPT = (TS >= 16) ? 2 : 1;
-  static constexpr int NT = TS * TS / PPT;   // threads per list split (one or two pixels each)
-  // list splits per block: blocks of 4 waves, as many as 7 waves per SIMD hold exactly (28
-  // per CU; 8-wave blocks fit only 3 times, and the phase-1 blend of the 16-view shell batch
+  static constexpr int NT = TS * TS / PPT;   // threads per group (one or two pixels each)
+  // splits: as many groups as make blocks of 4 waves (as many as 7 waves per SIMD hold exactly,
+  // 28 per CU; 8-wave blocks fit only 3 times, and the phase-1 blend of the 16-view shell batch
   // took 1.10 ms with 4 splits of 2 waves against 0.89 unsplit).  The phase-2 lists are short
-  // (its blend is bound by loading and storing the pixel state), so it keeps one group, as do
+  // (its blend is bound by loading and storing the pixel state), so it is not split, nor are
   // 32-pixel tiles (8 waves already).
   static constexpr int KS = (PHASE == 2 || NT >= 256) ? 1 : 256 / NT;
-  static constexpr int NB = NT * KS;         // threads per block
+  static constexpr int nb(bool split) { return NT * (split ? KS : 1); }   // threads per block
 };
 
-// The phase-2 blend of 16-pixel tiles is held to 128 VGPRs (4 waves per SIMD instead of 3):
-// its tiles are short and its time is mostly loading and storing the per-pixel state, which
-// more resident waves hide better (0.112 -> 0.101 ms).
-template <int TS, int GB, int PHASE>
-__global__ __launch_bounds__((BlendShape<TS, PHASE>::NB), ((TS == 16) ? (PHASE == 1 ? 7 : (PHASE == 2 ? 4 : 1)) : 1))
+// Waves per SIMD the blend of 16-pixel tiles is compiled for.  The phase-2 blend is held to 128
+// VGPRs (4 waves instead of 3): its tiles are short and its time is mostly loading and storing
+// the per-pixel state, which more resident waves hide better (0.112 -> 0.101 ms).  Phase 1 is
+// held to 7, except where LDS caps it first: the 2-wave blocks of larger LDS batches (staging
+// plus the merge buffer: 14 KB for GB = 128 and 22 KB for 256) fit 6 and 4 waves per SIMD.
+template <int TS, int GB, int PHASE, bool SPLIT>
+constexpr int blend_waves() {
+  if (TS != 16) return 1;
+  if (PHASE == 2) return 4;
+  if (PHASE == 0) return 1;
+  return (SPLIT || GB <= 64) ? 7 : (GB == 128 ? 6 : 4);
+}
+
+template <int TS, int GB, int PHASE, bool SPLIT>
+__global__ __launch_bounds__((BlendShape<TS, PHASE>::nb(SPLIT)), (blend_waves<TS, GB, PHASE, SPLIT>()))
 void k_blend(BlendArgs A) {
   constexpr int PPT = BlendShape<TS, PHASE>::PPT;
   constexpr int NT = BlendShape<TS, PHASE>::NT;
   constexpr int KS = BlendShape<TS, PHASE>::KS;
-  constexpr int NB = BlendShape<TS, PHASE>::NB;
+  constexpr int KP = SPLIT ? KS : 1;     // groups side by side
+  constexpr int NB = NT * KP;
   // records per online-max update: 2 pairs per thread in the first pass, 4 in the others.
   // Fewer pairs in flight need fewer registers (70 and 88 VGPRs for 16-pixel tiles instead of
   // 112 and 128), and the extra resident waves hide the LDS and transcendental latencies: with
   // 8 pairs the first pass took 0.88 ms, with 2 it takes 0.82-0.83 ms, and the second pass
   // 0.093 -> 0.087 ms with 4.
   constexpr int SUB = (PHASE == 1) ? 2 / PPT : 4 / PPT;
+  constexpr int R = GB / KP;             // records staged per group and round
   static_assert(GB % (KS * SUB) == 0, "a round's slices fit the LDS batch");
+  static_assert((R & (R - 1)) == 0, "slices of a power-of-two size");
   __shared__ float4 lds[GB * 4];
   __shared__ float sh_red[NB / 64 + 1];
   __shared__ float mrg[(KS > 1 ? KS - 1 : 1) * 6][NT * PPT];   // splits 1..KS-1: M S Z E Zall nan per pixel
-  const int split = __builtin_amdgcn_readfirstlane((int)threadIdx.x / NT);   // list split (wave-uniform)
-  const int ti = (int)threadIdx.x - split * NT;    // its pixel thread within the split
+  const int split = (KP > 1) ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / NT) : 0;   // group (wave-uniform)
+  const int ti = (int)threadIdx.x - split * NT;    // its pixel thread within the group
 
   // one block per tile
   const int64_t gt = A.tord ? (int64_t)A.tord[blockIdx.x] : xcd_tile(blockIdx.x, gridDim.x);
@@ -1077,7 +1096,7 @@ void k_blend(BlendArgs A) {
   bool nanf[PPT];
 #pragma unroll
   for (int q = 0; q < PPT; ++q) {
-    if (PHASE == 2 && split == 0) {   // the other splits start empty and are merged into it
+    if (PHASE == 2) {   // (not split)
       M[q] = A.st[jb.pix_base + lpx[q]];
       S[q] = A.st[A.total_pix + jb.pix_base + lpx[q]];
       Z[q] = A.st[2 * A.total_pix + jb.pix_base + lpx[q]];
@@ -1090,7 +1109,7 @@ void k_blend(BlendArgs A) {
       M[q] = -3.402823466e38f; S[q] = 0.f; Z[q] = 0.f; E[q] = 0.f; Zall[q] = 0.f;
       nanf[q] = false;
     }
-    Mt[q] = (r2 && PHASE == 2 && split == 0) ? M[q] / b3 : -3.402823466e38f;
+    Mt[q] = (r2 && PHASE == 2) ? M[q] / b3 : -3.402823466e38f;
   }
 
   int64_t start, end;
@@ -1099,81 +1118,97 @@ void k_blend(BlendArgs A) {
   else { start = (int64_t)A.off2[gt]; end = (int64_t)A.off2[gt + 1]; }
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
   const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
-
-  for (int64_t base = start; base < end; base += GB) {
-    const int nb = (int)min<int64_t>(GB, end - base);
-    const int sq = (nb + KS * SUB - 1) / (KS * SUB) * SUB;   // records per split this round
-    const int nbr = sq * KS;
-    if (r2) {
-      // 3 float4 per record; null record: X = 0, Y1 = 1, wL = -inf -> contributes nothing
-      for (int i = threadIdx.x; i < nbr * 3; i += NB) {
-        const int k = i / 3, part = i - 3 * k;
-        float4 v;
-        if (k < nb) {
-          const int64_t gid = (PHASE == 0) ? base + k : (int64_t)A.list[base + k];
-          v = rec24[gid * 4 + part];
-        } else {
-          v = (part == 1) ? make_float4(0.f, 1.f, 0.f, 0.f)
-              : (part == 2) ? make_float4(0.f, -INFINITY, 0.f, 0.f)
-                            : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        lds[i] = v;
-      }
-      __syncthreads();
-      const float4* sl = lds + split * sq * 3;   // this split's slice
-      if (PPT > 1 && !jb.ray_xy) {
-        if (masked) blend_batch_rec2<SUB, true, PPT, true>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-        else blend_batch_rec2<SUB, false, PPT, true>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-      } else {
-        if (masked) blend_batch_rec2<SUB, true, PPT, false>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-        else blend_batch_rec2<SUB, false, PPT, false>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
-      }
-      __syncthreads();
-      continue;
-    }
-    for (int i = threadIdx.x; i < nbr * 4; i += NB) {
-      const int k = i >> 2, part = i & 3;
-      float4 v;
-      if (k < nb) {
-        const int64_t gid = (PHASE == 0) ? base + k : (int64_t)A.list[base + k];
-        v = rec4[gid * 4 + part];
+  // (a tile lists each record of its view at most once: its list length fits 32 bits)
+  const int L = (int)(end - start);
+  const int h = ((L + KS - 1) / KS + SUB - 1) / SUB * SUB;   // records per split (the last may hold fewer)
+  auto range_len = [&](int v) { return min(L, (v + 1) * h) - min(L, v * h); };
+  // stage this round's records (P float4 each) of the groups' ranges into their LDS slots,
+  // padded with null records to the slot's last SUB group
+  auto stage = [&](auto pc, int v, int o) {
+    constexpr int P = decltype(pc)::value;
+    for (int i = threadIdx.x; i < KP * R * P; i += NB) {
+      const int k = i / P, part = i - P * k;
+      const int g = (KP > 1) ? k / R : 0, kk = k - g * R;   // group slot, record within it
+      const int vg = (KP > 1) ? g : v;
+      const int cg = min(R, range_len(vg) - o);   // the slot's records this round (may be <= 0)
+      if (kk >= (cg + SUB - 1) / SUB * SUB) continue;
+      float4 val;
+      if (kk < cg) {
+        const int li = vg * h + o + kk;
+        const int64_t gid = (PHASE == 0) ? (int64_t)li : (int64_t)A.list[start + li];
+        val = (P == 3) ? rec24[gid * 4 + part] : rec4[gid * 4 + part];
+      } else if (P == 3) {
+        // null record: X = 0, Y1 = 1, wL = -inf -> contributes nothing
+        val = (part == 1) ? make_float4(0.f, 1.f, 0.f, 0.f)
+            : (part == 2) ? make_float4(0.f, -INFINITY, 0.f, 0.f)
+                          : make_float4(0.f, 0.f, 0.f, 0.f);
       } else {
         // null record: u = (1,0,0), q = 0, w = -inf -> contributes nothing
-        v = (part == 1) ? make_float4(0.f, 0.f, 1.f, 0.f)
+        val = (part == 1) ? make_float4(0.f, 0.f, 1.f, 0.f)
             : (part == 3) ? make_float4(-INFINITY, -INFINITY, 0.f, 0.f)
                           : make_float4(0.f, 0.f, 0.f, 0.f);
       }
-      lds[i] = v;
+      lds[i] = val;
     }
-    __syncthreads();
+  };
+  const int parts = r2 ? 3 : 4;   // float4 per record
+
+  for (int pass = 0; pass < KS / KP; ++pass) {
+    // side by side: group `split` blends split `split`; one after another: splits 1, ..., KS-1,
+    // then 0, whose state stays in registers for the merge
+    const int v = (KP > 1) ? split : (pass + 1) % KS;
+    const int len = range_len(v);
+    const int span = (KP > 1) ? range_len(0) : len;   // rounds of the block (split 0 is the longest)
+    for (int o = 0; o < span; o += R) {
+      if (r2) stage(std::integral_constant<int, 3>{}, v, o);
+      else stage(std::integral_constant<int, 4>{}, v, o);
+      __syncthreads();
+      const int cnt = max(0, min(R, len - o));
+      const int sq = (cnt + SUB - 1) / SUB * SUB;
+      const float4* sl = lds + split * R * parts;   // this group's slot
+      if (r2) {
+        if (PPT > 1 && !jb.ray_xy) {
+          if (masked) blend_batch_rec2<SUB, true, PPT, true>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+          else blend_batch_rec2<SUB, false, PPT, true>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        } else {
+          if (masked) blend_batch_rec2<SUB, true, PPT, false>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+          else blend_batch_rec2<SUB, false, PPT, false>(sl, sq, xf, yf, b3, kap, M, Mt, S, Z, E, Zall);
+        }
+      } else {
 #pragma unroll
-    for (int q = 0; q < PPT; ++q) {
-      const float4* sl = lds + split * sq * 4;
-      if (safe) blend_batch<SUB, true>(sl, sq, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
-      else blend_batch<SUB, false>(sl, sq, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
+        for (int q = 0; q < PPT; ++q) {
+          if (safe) blend_batch<SUB, true>(sl, sq, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
+          else blend_batch<SUB, false>(sl, sq, xf[q], yf[q], b3, nb2l, kd, M[q], S[q], Z[q], E[q], Zall[q], nanf[q]);
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
+    // splits 1..KS-1 leave their states for the merge (one after another: in this thread's own
+    // entries, which it reads back itself, then it starts the next split empty)
+    if (KS > 1 && v > 0) {
+#pragma unroll
+      for (int q = 0; q < PPT; ++q) {
+        const int pi = q * NT + ti;   // consecutive lanes, consecutive words
+        float* m = &mrg[(v - 1) * 6][0];
+        m[pi] = M[q];
+        m[NT * PPT + pi] = S[q];
+        m[2 * NT * PPT + pi] = Z[q];
+        m[3 * NT * PPT + pi] = E[q];
+        m[4 * NT * PPT + pi] = Zall[q];
+        m[5 * NT * PPT + pi] = nanf[q] ? 1.0f : 0.0f;
+        M[q] = -3.402823466e38f; S[q] = 0.f; Z[q] = 0.f; E[q] = 0.f; Zall[q] = 0.f;
+        nanf[q] = false;
+        Mt[q] = -3.402823466e38f;
+      }
+    }
   }
 
   // merge the splits' states into split 0: the maximum, S and Z rescaled to it, E, Zall added
-  if (KS > 1 && split > 0) {
-#pragma unroll
-    for (int q = 0; q < PPT; ++q) {
-      const int pi = ti * PPT + q;
-      float* m = &mrg[(split - 1) * 6][0];
-      m[pi] = M[q];
-      m[NT * PPT + pi] = S[q];
-      m[2 * NT * PPT + pi] = Z[q];
-      m[3 * NT * PPT + pi] = E[q];
-      m[4 * NT * PPT + pi] = Zall[q];
-      m[5 * NT * PPT + pi] = nanf[q] ? 1.0f : 0.0f;
-    }
-  }
-  if (KS > 1) __syncthreads();
+  if (KP > 1) __syncthreads();
   if (KS > 1 && split == 0) {
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
-      const int pi = ti * PPT + q;
+      const int pi = q * NT + ti;
       float Mm = M[q];
 #pragma unroll
       for (int k = 1; k < KS; ++k) Mm = fmaxf(Mm, mrg[(k - 1) * 6][pi]);
@@ -1375,12 +1410,50 @@ hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t
   return hipGetLastError();
 }
 
+// CUs of the current device (cached per device; 256 if the query fails)
+static int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int c = cache[dev].load(std::memory_order_relaxed);
+  if (c <= 0) {
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cache[dev].store(c, std::memory_order_relaxed);
+  }
+  return c;
+}
+
+// Where a split tile list runs (the same arithmetic either way).  Side by side halves a block's
+// latency, which pays wherever the grid's last blocks set the time: measured phase-1 blends of
+// N=4096 views, side by side against one after another, 16-pixel tiles: one 128x128 view 0.35
+// against 0.74 ms, 64 such views 0.41 against 0.74, 8 480x640 views 0.43 against 0.44, and the
+// 16-view 480x640 shell (the grid 11 times what the GPU holds at once) 0.95 against 0.92 under
+// sustained load.  8-pixel tiles run their 4 one-wave groups side by side always (the 16-view
+// shell: 0.88 against 0.91 ms).
+template <int TS, int PHASE>
+static bool blend_side_by_side(int64_t total_tiles) {
+  if (TS == 8) return true;
+  constexpr int waves = BlendShape<TS, PHASE>::nb(true) / 64;
+  return total_tiles * waves <= (int64_t)8 * device_cus() * 28;   // 28 waves per CU at once
+}
+
+template <int TS, int GB, int PHASE>
+static void launch_blend_p(hipStream_t s, int64_t total_tiles, const BlendArgs& a) {
+  const dim3 grid((unsigned)total_tiles);
+  if constexpr (BlendShape<TS, PHASE>::KS > 1) {
+    if (blend_side_by_side<TS, PHASE>(total_tiles)) {
+      hipLaunchKernelGGL((k_blend<TS, GB, PHASE, true>), grid, dim3(BlendShape<TS, PHASE>::nb(true)), 0, s, a);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((k_blend<TS, GB, PHASE, false>), grid, dim3(BlendShape<TS, PHASE>::nb(false)), 0, s, a);
+}
+
 template <int TS, int GB>
 static hipError_t launch_blend_tg(hipStream_t s, int phase, int64_t total_tiles, const BlendArgs& a) {
-  const dim3 grid((unsigned)total_tiles);
-  if (phase == 0) hipLaunchKernelGGL((k_blend<TS, GB, 0>), grid, dim3(BlendShape<TS, 0>::NB), 0, s, a);
-  else if (phase == 1) hipLaunchKernelGGL((k_blend<TS, GB, 1>), grid, dim3(BlendShape<TS, 1>::NB), 0, s, a);
-  else hipLaunchKernelGGL((k_blend<TS, GB, 2>), grid, dim3(BlendShape<TS, 2>::NB), 0, s, a);
+  if (phase == 0) launch_blend_p<TS, GB, 0>(s, total_tiles, a);
+  else if (phase == 1) launch_blend_p<TS, GB, 1>(s, total_tiles, a);
+  else launch_blend_p<TS, GB, 2>(s, total_tiles, a);
   return hipGetLastError();
 }
 
