@@ -52,7 +52,7 @@ struct Layout {
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_rec2 = 0, o_tnp = 0,
          o_mask = 0, o_count = 0,
          o_off1 = 0, o_off2 = 0, o_partial = 0, o_list = 0, o_lmin = 0, o_ubm = 0, o_state = 0,
-         o_bjobs = 0, o_gsum = 0, o_rect = 0, o_order = 0, o_rank = 0, o_tord = 0, bytes = 0;
+         o_bjobs = 0, o_gsum = 0, o_rect = 0, o_order = 0, o_rank = 0, o_tord = 0, o_mpart = 0, bytes = 0;
 };
 
 // A small header at the start of the workspace lets fe_read_stats find the counters.
@@ -230,6 +230,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     L->o_off2 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
     L->o_partial = p; p = align256(p + sizeof(unsigned long long) * (scan_partial_count(tiles) + 1));
     L->o_tord = p; p = align256(p + sizeof(int32_t) * 2 * tiles);   // phase-1 order, then union order
+    L->o_mpart = p; p = align256(p + sizeof(uint32_t) * 4 * kMaskMaxSub * kMaskChunkBlocks);   // membership partials
     L->o_list = p; p = align256(p + sizeof(uint32_t) * (size_t)std::max<int64_t>(cap, 1));
   }
   if (L->save) {
@@ -430,10 +431,11 @@ static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_s
   uint32_t* tnp = at<uint32_t>(ws, L.o_tnp);
   int32_t* tord = at<int32_t>(ws, L.o_tord);   // [0, T): phase-1 blend order; [T, 2T): backward order
   uint32_t* thist = at<uint32_t>(ws, L.o_counters + offsetof(Counters, tile_hist));
+  uint32_t* mpart = at<uint32_t>(ws, L.o_mpart);
   // phase 1
   FE_HIP(timed(s, FE_K_MASK1, [&] {
-    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, seed, ubm, tnp,
-                            flags, cnt);
+    return launch_tile_mask(s, 1, dj, L.n_jobs, L.total_stiles, L.ts, L.max_n, rec, mask, mask2, count, lmin, seed,
+                            ubm, tnp, flags, cnt, mpart);
   }, may_time));
   FE_HIP(timed(s, FE_K_SCAN1, [&] {
     return launch_scan(s, count, L.total_tiles, partial, off1, nullptr, nullptr, thist, tord);
@@ -446,8 +448,8 @@ static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_s
   }, may_time));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
-    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, rec, mask, mask2, count, lmin, seed, ubm, tnp,
-                            flags, cnt);
+    return launch_tile_mask(s, 2, dj, L.n_jobs, L.total_stiles, L.ts, L.max_n, rec, mask, mask2, count, lmin, seed,
+                            ubm, tnp, flags, cnt, mpart);
   }, may_time));
   FE_HIP(timed(s, FE_K_SCAN2, [&] {
     // the backward's order: both lists of a tile (phase-1 length from off1)

@@ -154,6 +154,13 @@ constexpr int kGsumPlanes = 11;
 // counter would take one same-address atomic per blend block (19k per 16-view launch), and
 // those serialise at memory (they bounded the phase-2 blend).  Readers add the slots.
 constexpr int kCountSlots = 64;
+// Membership of a small grid splits each group's Gaussians over several blocks (at most
+// kMaskChunkBlocks blocks, for at most kMaskChunkGroups groups): each block leaves its per-tile
+// partial count, flag, bound and threshold (4 words) in a workspace region, and a second kernel
+// merges them and finishes the group's tiles.
+constexpr int kMaskChunkBlocks = 1024;
+constexpr int kMaskChunkGroups = kMaskChunkBlocks / 2;
+constexpr int kMaskMaxSub = 16;   // tiles per group at most (4x4 8-pixel tiles)
 struct Counters {
   unsigned long long floor_tiles;       // tiles completed densely because the floor could matter
   unsigned long long floor_unresolved;  // pixels' tiles where it still could (must stay 0)

@@ -311,6 +311,42 @@ __global__ __launch_bounds__(256) void k_preprocess(const Job* __restrict__ jobs
 // ------------------------------------------------------------------------------------------
 constexpr int kWinWords = 64;   // membership words per LDS window
 
+// Finish one tile of a group (one full wave): its pair count and not-known-in-front flag; in
+// phase 2 also the largest masked-logit bound `ubm` (with the phase-1 floor seed), and the dense
+// completion of the tile when the floor could become visible.
+template <int PHASE>
+__device__ void finish_tile(const Job& jb, int64_t tile, uint32_t total, uint32_t np, float ubm, float lmin,
+                            int lane, const unsigned long long* __restrict__ mask1,
+                            unsigned long long* __restrict__ mask2, uint32_t* __restrict__ count,
+                            const float* __restrict__ tile_seed, float* __restrict__ tile_ubm,
+                            uint32_t* __restrict__ tile_np, Counters* __restrict__ cnt_out) {
+  const int64_t gt = jb.tile_base + tile;
+  if (PHASE == 2) {
+    const float ubm_t = fmaxf(ubm, tile_seed[gt]);
+    const bool dense_tile = (ubm_t - lmin >= jb.floor_thr);   // false when nothing in doubt
+    if (dense_tile) {
+      // complete the tile: every pair not in phase 1
+      const int64_t row = jb.mask_base + tile * jb.nw;
+      uint32_t c2 = 0;
+      for (int c = 0; c < jb.nw; ++c) {
+        const int j = c * 64 + lane;
+        const bool keep = (j < jb.n) && !((mask1[row + c] >> lane) & 1ull);
+        const unsigned long long word = __ballot(keep);
+        if (lane == 0) mask2[row + c] = word;
+        c2 += (uint32_t)__popcll(word);
+      }
+      total = c2;
+      np = 1;   // completed without bounds: none is known to be in front
+      if (lane == 0) atomicAdd(&cnt_out->floor_tiles, 1ull);
+    }
+    if (lane == 0) tile_ubm[gt] = dense_tile ? -INFINITY : ubm_t;
+  }
+  if (lane == 0) {
+    count[gt] = total;
+    tile_np[2 * gt + (PHASE - 1)] = np;
+  }
+}
+
 // Phase 2 with 3x3 groups is held to 128 VGPRs (4 waves per SIMD instead of 3; a few
 // registers spill): 0.171 -> 0.152 ms.  Phase 1 at 5 waves (96 VGPRs) spilled into a slower
 // kernel.
@@ -325,8 +361,10 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
                                                    float* __restrict__ tile_ubm,
                                                    uint32_t* __restrict__ tile_np,
                                                    const int32_t* __restrict__ job_flags,
-                                                   Counters* __restrict__ cnt_out) {
+                                                   Counters* __restrict__ cnt_out, int chunks,
+                                                   uint32_t* __restrict__ mpart) {
   constexpr int NS = SFMAX * SFMAX;   // sub-tiles per group at most (ray-list views use 1)
+  static_assert(NS <= kMaskMaxSub, "partials hold kMaskMaxSub tiles per group");
   __shared__ float sh_ubm[4][NS];
   const int64_t gs = blockIdx.x;
   const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
@@ -408,9 +446,12 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
 #pragma unroll
   for (int k = 0; k < NS; ++k) ubm[k] = -INFINITY;
   const bool single = (nsub == 1);   // the group is the tile: its bound decides directly
+  // this block's share of the Gaussians (chunks > 1 on small grids): words [wlo, whi) of 64
+  const int cw = (jb.nw + chunks - 1) / chunks;
+  const int wlo = min(jb.nw, (int)blockIdx.y * cw), whi = min(jb.nw, wlo + cw);
 
-  for (int w0 = 0; w0 < jb.nw; w0 += kWinWords) {
-    const int nww = min(kWinWords, jb.nw - w0);
+  for (int w0 = wlo; w0 < whi; w0 += kWinWords) {
+    const int nww = min(kWinWords, whi - w0);
     for (int i = threadIdx.x; i < NS * kWinWords; i += 256) {
       const int k = i / kWinWords, c = i - k * kWinWords;
       wmask[k][c] = 0ull;
@@ -568,39 +609,61 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
     if (lane == 0) sh_ubm[wave][k] = ubm[k];
   }
   __syncthreads();
+  if (chunks > 1) {
+    // leave this block's per-tile partials (count, flag, bound, threshold); k_mask_finish
+    // merges them (sum, or, max: the same totals in any order) and finishes the tiles
+    if (threadIdx.x < nsub) {
+      const int k = threadIdx.x;
+      uint32_t* mp = mpart + ((int64_t)gs * chunks + blockIdx.y) * (4 * kMaskMaxSub) + 4 * k;
+      mp[0] = tcnt[k];
+      mp[1] = tnp[k];
+      mp[2] = __float_as_uint(fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k])));
+      mp[3] = __float_as_uint(PHASE == 2 ? lmin[k] : 0.0f);
+    }
+    return;
+  }
   // one wave finishes the sub-tiles
   if (wave == 0) {
     for (int k = 0; k < nsub; ++k) {
       const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
-      const int64_t gt = jb.tile_base + tile;
-      uint32_t total = tcnt[k];
-      uint32_t np = tnp[k];
-      if (PHASE == 2) {
-        const float ubm_t = fmaxf(fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k])),
-                                  tile_seed[gt]);
-        const bool dense_tile = (ubm_t - lmin[k] >= jb.floor_thr);   // false when nothing in doubt
-        if (dense_tile) {
-          // complete the tile: every pair not in phase 1
-          const int64_t row = jb.mask_base + tile * jb.nw;
-          uint32_t c2 = 0;
-          for (int c = 0; c < jb.nw; ++c) {
-            const int j = c * 64 + lane;
-            const bool keep = (j < jb.n) && !((mask1[row + c] >> lane) & 1ull);
-            const unsigned long long word = __ballot(keep);
-            if (lane == 0) mask2[row + c] = word;
-            c2 += (uint32_t)__popcll(word);
-          }
-          total = c2;
-          np = 1;   // completed without bounds: none is known to be in front
-          if (lane == 0) atomicAdd(&cnt_out->floor_tiles, 1ull);
-        }
-        if (lane == 0) tile_ubm[gt] = dense_tile ? -INFINITY : ubm_t;
-      }
-      if (lane == 0) {
-        count[gt] = total;
-        tile_np[2 * gt + (PHASE - 1)] = np;
-      }
+      const float u = fmaxf(fmaxf(sh_ubm[0][k], sh_ubm[1][k]), fmaxf(sh_ubm[2][k], sh_ubm[3][k]));
+      finish_tile<PHASE>(jb, tile, tcnt[k], tnp[k], u, lmin[k], lane, mask1, mask2, count, tile_seed, tile_ubm,
+                         tile_np, cnt_out);
     }
+  }
+}
+
+// Merges the partials of a group's chunks (k_tile_mask with chunks > 1) and finishes its
+// tiles: one wave per group.
+template <int PHASE>
+__global__ __launch_bounds__(64) void k_mask_finish(const Job* __restrict__ jobs, int n_jobs, int chunks,
+                                                    const uint32_t* __restrict__ mpart,
+                                                    const unsigned long long* __restrict__ mask1,
+                                                    unsigned long long* __restrict__ mask2,
+                                                    uint32_t* __restrict__ count, const float* __restrict__ tile_seed,
+                                                    float* __restrict__ tile_ubm, uint32_t* __restrict__ tile_np,
+                                                    Counters* __restrict__ cnt_out) {
+  const int64_t gs = blockIdx.x;
+  const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
+  const Job& jb = jobs[ji];
+  const int sf = jb.sf;
+  const int st = (int)(gs - jb.stile_base);
+  const int sx = st % jb.stiles_x, sy = st / jb.stiles_x;
+  const int tx0 = sx * sf, ty0 = sy * sf;
+  const int nsx = min(sf, jb.tiles_x - tx0), nsy = min(sf, jb.tiles_y - ty0);
+  const int lane = threadIdx.x;
+  for (int k = 0; k < nsx * nsy; ++k) {
+    uint32_t total = 0, np = 0;
+    float u = -INFINITY;
+    for (int q = 0; q < chunks; ++q) {
+      const uint32_t* mp = mpart + ((int64_t)gs * chunks + q) * (4 * kMaskMaxSub) + 4 * k;
+      total += mp[0];
+      np |= mp[1];
+      u = fmaxf(u, __uint_as_float(mp[2]));
+    }
+    const float lm = __uint_as_float(mpart[(int64_t)gs * chunks * (4 * kMaskMaxSub) + 4 * k + 3]);
+    const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
+    finish_tile<PHASE>(jb, tile, total, np, u, lm, lane, mask1, mask2, count, tile_seed, tile_ubm, tile_np, cnt_out);
   }
 }
 
@@ -1332,6 +1395,19 @@ void k_blend(BlendArgs A) {
 
 namespace fe {
 
+// CUs of the current device (cached per device; 256 if the query fails)
+static int device_cus() {
+  static std::atomic<int> cache[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  int c = cache[dev].load(std::memory_order_relaxed);
+  if (c <= 0) {
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cache[dev].store(c, std::memory_order_relaxed);
+  }
+  return c;
+}
+
 hipError_t launch_init(hipStream_t s, Counters* cnt, int32_t* job_flags, int n_jobs) {
   const int blocks = max(1, min(64, (n_jobs + 255) / 256));
   hipLaunchKernelGGL(k_init, dim3(blocks), dim3(256), 0, s, cnt, job_flags, n_jobs);
@@ -1366,26 +1442,39 @@ hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max
 }
 
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
-                            int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
+                            int ts, int max_n, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
                             uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
-                            uint32_t* tile_np, const int32_t* job_flags, Counters* cnt) {
-  // the group factor is 4 for 8-pixel tiles, 3 for 16, 1 for 32 and for ray lists
-  const dim3 g((unsigned)total_stiles), b(256);
-  if (phase == 1) {
-    if (ts == 8) hipLaunchKernelGGL((k_tile_mask<1, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                                    tile_seed, tile_ubm, tile_np, job_flags, cnt);
-    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<1, 3>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
-                                          tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt);
-    else hipLaunchKernelGGL((k_tile_mask<1, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                            tile_seed, tile_ubm, tile_np, job_flags, cnt);
-  } else {
-    if (ts == 8) hipLaunchKernelGGL((k_tile_mask<2, 4>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                                    tile_seed, tile_ubm, tile_np, job_flags, cnt);
-    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 3>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count,
-                                          tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt);
-    else hipLaunchKernelGGL((k_tile_mask<2, 1>), g, b, 0, s, jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin,
-                            tile_seed, tile_ubm, tile_np, job_flags, cnt);
+                            uint32_t* tile_np, const int32_t* job_flags, Counters* cnt, uint32_t* mpart) {
+  if (total_stiles <= 0) return hipSuccess;
+  // A grid of few groups leaves most of the GPU idle while each block walks all n Gaussians:
+  // up to 4 blocks per CU then share each group's Gaussians, in chunks of at least 4 words (256
+  // Gaussians), and k_mask_finish merges their partials.  (One-tile groups for small views
+  // instead: one 128x128 view, N=4096, 235 -> 49 us for both phases, but the 256-scene batch
+  // 8% slower; chunks leave the groups, and the lists, as they are.)
+  int chunks = 1;
+  if (total_stiles <= kMaskChunkGroups) {
+    const int64_t want = (int64_t)4 * device_cus() / total_stiles;
+    const int64_t cap = std::min<int64_t>(kMaskChunkBlocks / total_stiles, std::max(1, ((max_n + 63) / 64) / 4));
+    chunks = (int)std::max<int64_t>(1, std::min(want, cap));
   }
+  // the group factor is 4 for 8-pixel tiles, 3 for 16, 1 for 32 and for ray lists
+  const dim3 g((unsigned)total_stiles, (unsigned)chunks), b(256);
+#define FE_MASK_ARGS jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt, \
+                     chunks, mpart
+#define FE_FINISH_ARGS jobs, n_jobs, chunks, mpart, mask1, mask2, count, tile_seed, tile_ubm, tile_np, cnt
+  if (phase == 1) {
+    if (ts == 8) hipLaunchKernelGGL((k_tile_mask<1, 4>), g, b, 0, s, FE_MASK_ARGS);
+    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<1, 3>), g, b, 0, s, FE_MASK_ARGS);
+    else hipLaunchKernelGGL((k_tile_mask<1, 1>), g, b, 0, s, FE_MASK_ARGS);
+    if (chunks > 1) hipLaunchKernelGGL(k_mask_finish<1>, dim3((unsigned)total_stiles), dim3(64), 0, s, FE_FINISH_ARGS);
+  } else {
+    if (ts == 8) hipLaunchKernelGGL((k_tile_mask<2, 4>), g, b, 0, s, FE_MASK_ARGS);
+    else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 3>), g, b, 0, s, FE_MASK_ARGS);
+    else hipLaunchKernelGGL((k_tile_mask<2, 1>), g, b, 0, s, FE_MASK_ARGS);
+    if (chunks > 1) hipLaunchKernelGGL(k_mask_finish<2>, dim3((unsigned)total_stiles), dim3(64), 0, s, FE_FINISH_ARGS);
+  }
+#undef FE_MASK_ARGS
+#undef FE_FINISH_ARGS
   return hipGetLastError();
 }
 
@@ -1408,19 +1497,6 @@ hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t
   hipLaunchKernelGGL(k_tile_write, dim3((unsigned)((total_tiles + 3) / 4)), dim3(256), 0, s, jobs, n_jobs,
                      total_tiles, mask, off, list);
   return hipGetLastError();
-}
-
-// CUs of the current device (cached per device; 256 if the query fails)
-static int device_cus() {
-  static std::atomic<int> cache[64];
-  int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  int c = cache[dev].load(std::memory_order_relaxed);
-  if (c <= 0) {
-    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
-    cache[dev].store(c, std::memory_order_relaxed);
-  }
-  return c;
 }
 
 // Where a split tile list runs (the same arithmetic either way).  Side by side halves a block's

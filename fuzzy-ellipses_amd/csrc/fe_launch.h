@@ -25,10 +25,11 @@ hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t 
 hipError_t launch_order(hipStream_t s, const Job* jobs, int n_jobs, int max_n, Counters* cnt, int32_t* job_flags);
 hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec, float* rec2,
                              int32_t* job_flags);
+// mpart: the membership partials region (4 * kMaskMaxSub * kMaskChunkBlocks words)
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
-                            int ts, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
+                            int ts, int max_n, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
                             uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
-                            uint32_t* tile_np, const int32_t* job_flags, Counters* cnt);
+                            uint32_t* tile_np, const int32_t* job_flags, Counters* cnt, uint32_t* mpart);
 int64_t scan_partial_count(int64_t n);
 // With hist and tord (both or neither), also writes the tiles' dispatch order to tord (a
 // counting sort over kTileBuckets buckets of the tile key; hist must start at zero).
