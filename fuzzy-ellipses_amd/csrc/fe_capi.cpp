@@ -230,7 +230,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
     L->o_off2 = p; p = align256(p + sizeof(unsigned long long) * (tiles + 1));
     L->o_partial = p; p = align256(p + sizeof(unsigned long long) * (scan_partial_count(tiles) + 1));
     L->o_tord = p; p = align256(p + sizeof(int32_t) * 2 * tiles);   // phase-1 order, then union order
-    L->o_mpart = p; p = align256(p + sizeof(uint32_t) * 4 * kMaskMaxSub * kMaskChunkBlocks);   // membership partials
+    L->o_mpart = p; p = align256(p + sizeof(uint32_t) * 5 * kMaskMaxSub * kMaskChunkBlocks);   // membership partials, bounds
     L->o_list = p; p = align256(p + sizeof(uint32_t) * (size_t)std::max<int64_t>(cap, 1));
   }
   if (L->save) {

@@ -157,7 +157,8 @@ constexpr int kCountSlots = 64;
 // Membership of a small grid splits each group's Gaussians over several blocks (at most
 // kMaskChunkBlocks blocks, for at most kMaskChunkGroups groups): each block leaves its per-tile
 // partial count, flag, bound and threshold (4 words) in a workspace region, and a second kernel
-// merges them and finishes the group's tiles.
+// merges them and finishes the group's tiles; after them, one word per (block, tile) holds a
+// phase-2 lower bound for tiles that phase 1 left without a threshold.
 constexpr int kMaskChunkBlocks = 1024;
 constexpr int kMaskChunkGroups = kMaskChunkBlocks / 2;
 constexpr int kMaskMaxSub = 16;   // tiles per group at most (4x4 8-pixel tiles)

@@ -347,6 +347,65 @@ __device__ void finish_tile(const Job& jb, int64_t tile, uint32_t total, uint32_
   }
 }
 
+// Per lane, over the Gaussians of words [w0, w1) (lane = Gaussian, waves interleaved), the
+// largest lower bound b3 (w - sd_hi^2/2) - b2 t_ub of the logit at every pixel of tile tg of a
+// Gaussian entirely in front of the tile's rays; -inf if none.
+__device__ __forceinline__ float front_logit_lb(const Job& jb, const float* __restrict__ rec, const TileGeo& tg,
+                                                int w0, int w1, int wave, int lane, float b3l, float b2l) {
+  float lb = -INFINITY;
+  for (int c = w0 + wave; c < w1; c += 4) {
+    const int j = c * 64 + lane;
+    if (j >= jb.n) continue;
+    const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
+    float r[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 t4 = r4[i];
+      r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
+    }
+    if (__float_as_int(r[15]) & 1) continue;
+    const PairBound pb = pair_bound(r, tg);
+    if (pb.plus && !pb.unbounded)
+      lb = fmaxf(lb, fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
+  }
+  return lb;
+}
+
+// Phase 2 of a chunked membership (chunks > 1): for every tile of a group whose phase-1 lmin is
+// -inf, the bound front_logit_lb over this block's chunk of the Gaussians, to
+// lbpart[(group * chunks + chunk) * kMaskMaxSub + tile of the group] (k_tile_mask takes the max).
+__global__ __launch_bounds__(256) void k_mask_lb(const Job* __restrict__ jobs, int n_jobs, int ts,
+                                                 const float* __restrict__ rec, const float* __restrict__ tile_lmin,
+                                                 const int32_t* __restrict__ job_flags, int chunks,
+                                                 float* __restrict__ lbpart) {
+  const int64_t gs = blockIdx.x;
+  const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
+  const Job& jb = jobs[ji];
+  if ((job_flags[ji] & 1) != 0 || !bounds_apply(jb)) return;   // dense job: no thresholds
+  const int sf = jb.sf;
+  const int st = (int)(gs - jb.stile_base);
+  const int sx = st % jb.stiles_x, sy = st / jb.stiles_x;
+  const int tx0 = sx * sf, ty0 = sy * sf;
+  const int nsx = min(sf, jb.tiles_x - tx0), nsy = min(sf, jb.tiles_y - ty0);
+  const int cw = (jb.nw + chunks - 1) / chunks;
+  const int wlo = min(jb.nw, (int)blockIdx.y * cw), whi = min(jb.nw, wlo + cw);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
+  __shared__ float sh_lb[4];
+  for (int k = 0; k < nsx * nsy; ++k) {
+    const int txk = tx0 + k % nsx, tyk = ty0 + k / nsx;
+    if (!(tile_lmin[jb.tile_base + (int64_t)tyk * jb.tiles_x + txk] == -INFINITY)) continue;   // block-uniform
+    float lb = front_logit_lb(jb, rec, tile_geo_xy(jb, txk, tyk, ts), wlo, whi, wave, lane, b3l, b2l);
+    for (int o = 32; o > 0; o >>= 1) lb = fmaxf(lb, __shfl_xor(lb, o, 64));
+    if (lane == 0) sh_lb[wave] = lb;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      lbpart[((int64_t)gs * chunks + blockIdx.y) * kMaskMaxSub + k] =
+          fmaxf(fmaxf(sh_lb[0], sh_lb[1]), fmaxf(sh_lb[2], sh_lb[3]));
+    __syncthreads();
+  }
+}
+
 // Phase 2 with 3x3 groups is held to 128 VGPRs (4 waves per SIMD instead of 3; a few
 // registers spill): 0.171 -> 0.152 ms.  Phase 1 at 5 waves (96 VGPRs) spilled into a slower
 // kernel.
@@ -362,7 +421,8 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
                                                    uint32_t* __restrict__ tile_np,
                                                    const int32_t* __restrict__ job_flags,
                                                    Counters* __restrict__ cnt_out, int chunks,
-                                                   uint32_t* __restrict__ mpart) {
+                                                   uint32_t* __restrict__ mpart,
+                                                   const float* __restrict__ lbpart) {
   constexpr int NS = SFMAX * SFMAX;   // sub-tiles per group at most (ray-list views use 1)
   static_assert(NS <= kMaskMaxSub, "partials hold kMaskMaxSub tiles per group");
   __shared__ float sh_ubm[4][NS];
@@ -399,34 +459,31 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
     // lmin = -inf.  Its threshold then comes from bounds: any Gaussian entirely in front of
     // the tile's rays has logit >= lb = b3 (w - sd_hi^2/2) - b2 t_ub at every pixel, so the
     // log2-sum-exp is at least max_j lb_j there (and that Gaussian passes the test below).
-    __shared__ float sh_lb[4];
-    for (int k = 0; k < nsub; ++k) {
-      if (!(lmin[k] == -INFINITY)) continue;   // block-uniform
-      float lb = -INFINITY;
-      for (int c = wave; c < jb.nw; c += 4) {
-        const int j = c * 64 + lane;
-        if (j >= jb.n) continue;
-        const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
-        float r[16];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const float4 t4 = r4[i];
-          r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
-        }
-        if (__float_as_int(r[15]) & 1) continue;
-        const PairBound pb = pair_bound(r, tg[k]);
-        if (pb.plus && !pb.unbounded)
-          lb = fmaxf(lb, fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
-      }
-      for (int o = 32; o > 0; o >>= 1) lb = fmaxf(lb, __shfl_xor(lb, o, 64));
-      if (lane == 0) sh_lb[wave] = lb;
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        const float m = fmaxf(fmaxf(sh_lb[0], sh_lb[1]), fmaxf(sh_lb[2], sh_lb[3]));
+    if (chunks > 1) {
+      // the maximum of the chunks' maxima (k_mask_lb)
+      if (threadIdx.x < nsub && lmin[threadIdx.x] == -INFINITY) {
+        const int k = threadIdx.x;
+        float m = -INFINITY;
+        for (int q = 0; q < chunks; ++q) m = fmaxf(m, lbpart[((int64_t)gs * chunks + q) * kMaskMaxSub + k]);
         lmin[k] = m;
         tau[k] = m - jb.delta2;
       }
       __syncthreads();
+    } else {
+      __shared__ float sh_lb[4];
+      for (int k = 0; k < nsub; ++k) {
+        if (!(lmin[k] == -INFINITY)) continue;   // block-uniform
+        float lb = front_logit_lb(jb, rec, tg[k], 0, jb.nw, wave, lane, b3l, b2l);
+        for (int o = 32; o > 0; o >>= 1) lb = fmaxf(lb, __shfl_xor(lb, o, 64));
+        if (lane == 0) sh_lb[wave] = lb;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+          const float m = fmaxf(fmaxf(sh_lb[0], sh_lb[1]), fmaxf(sh_lb[2], sh_lb[3]));
+          lmin[k] = m;
+          tau[k] = m - jb.delta2;
+        }
+        __syncthreads();
+      }
     }
   }
   if (PHASE == 2 && threadIdx.x == 0) {
@@ -1459,8 +1516,10 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
   }
   // the group factor is 4 for 8-pixel tiles, 3 for 16, 1 for 32 and for ray lists
   const dim3 g((unsigned)total_stiles, (unsigned)chunks), b(256);
+  // (the chunks' lower bounds follow the partials in the region)
+  float* lbpart = reinterpret_cast<float*>(mpart + 4 * kMaskMaxSub * kMaskChunkBlocks);
 #define FE_MASK_ARGS jobs, n_jobs, ts, rec, mask1, mask2, count, tile_lmin, tile_seed, tile_ubm, tile_np, job_flags, cnt, \
-                     chunks, mpart
+                     chunks, mpart, lbpart
 #define FE_FINISH_ARGS jobs, n_jobs, chunks, mpart, mask1, mask2, count, tile_seed, tile_ubm, tile_np, cnt
   if (phase == 1) {
     if (ts == 8) hipLaunchKernelGGL((k_tile_mask<1, 4>), g, b, 0, s, FE_MASK_ARGS);
@@ -1468,6 +1527,7 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
     else hipLaunchKernelGGL((k_tile_mask<1, 1>), g, b, 0, s, FE_MASK_ARGS);
     if (chunks > 1) hipLaunchKernelGGL(k_mask_finish<1>, dim3((unsigned)total_stiles), dim3(64), 0, s, FE_FINISH_ARGS);
   } else {
+    if (chunks > 1) hipLaunchKernelGGL(k_mask_lb, g, b, 0, s, jobs, n_jobs, ts, rec, tile_lmin, job_flags, chunks, lbpart);
     if (ts == 8) hipLaunchKernelGGL((k_tile_mask<2, 4>), g, b, 0, s, FE_MASK_ARGS);
     else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 3>), g, b, 0, s, FE_MASK_ARGS);
     else hipLaunchKernelGGL((k_tile_mask<2, 1>), g, b, 0, s, FE_MASK_ARGS);

@@ -25,7 +25,7 @@ hipError_t launch_ray_rects(hipStream_t s, const Job* jobs, int n_jobs, int64_t 
 hipError_t launch_order(hipStream_t s, const Job* jobs, int n_jobs, int max_n, Counters* cnt, int32_t* job_flags);
 hipError_t launch_preprocess(hipStream_t s, const Job* jobs, int n_jobs, int max_n, float* rec, float* rec2,
                              int32_t* job_flags);
-// mpart: the membership partials region (4 * kMaskMaxSub * kMaskChunkBlocks words)
+// mpart: the membership partials region (5 * kMaskMaxSub * kMaskChunkBlocks words)
 hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_jobs, int64_t total_stiles,
                             int ts, int max_n, const float* rec, unsigned long long* mask1, unsigned long long* mask2,
                             uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
