@@ -27,14 +27,12 @@ FLOPS_FWD = 36                    # per evaluated (pixel, Gaussian) pair, blend 
 FLOPS_BWD = 93                    # per evaluated pair, blend backward incl. recompute
 TRANS_PER_PAIR = 3                # v_exp x2 + v_rcp per evaluated pair, forward and backward
 TRANS_PEAK_TOPS = 9.8             # 256 CUs x 4 SIMDs x 16 lanes / 4 (quarter rate) x 2.4 GHz
-
-
-
 HBM_PEAK_GBS = 8000.0             # MI355X published HBM3E bandwidth
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "traffic_shell_n4096_16x480x640.json")
-PROFILE_NAMES = {"bwd_blend": "fe::k_bwd_blend<16>", "blend1": "fe::k_blend<16, 64, 1>",
-                 "blend2": "fe::k_blend<16, 64, 2>"}
+# kernel names in the profile (the 16-view workload's blends run their list splits one after another)
+PROFILE_NAMES = {"bwd_blend": "fe::k_bwd_blend<16>", "blend1": "fe::k_blend<16, 64, 1, false>",
+                 "blend2": "fe::k_blend<16, 64, 2, false>"}
 
 
 def profiled_counters(dom, args):
