@@ -1,9 +1,10 @@
 """Split tile lists: the blend cuts each tile list into fixed contiguous ranges (two on 16-pixel
-tiles, four on 8-pixel tiles), blends them from separate states and merges the states.  On 16-pixel tiles the launcher runs the ranges side by side
-for grids up to 8 times what the GPU holds at once (28 waves per CU; 57344 waves on 256 CUs)
-and one after another beyond that (8-pixel tiles always side by side).  The arithmetic is the
-same either way, so a view rendered inside a batch past that size equals the view rendered
-alone (a grid far below it), bit for bit."""
+tiles, four on 8-pixel tiles), blends them from separate states and merges the states.  On
+16-pixel tiles the launcher runs the ranges side by side for grids up to 8 times what the GPU
+holds at once (28 waves per CU; 57344 waves on 256 CUs) and one after another beyond that
+(8-pixel tiles always side by side).  The arithmetic is the same either way, so a view rendered
+inside a batch past that size equals the view rendered alone (a grid far below it), bit for
+bit."""
 import numpy as np
 import pytest
 
