@@ -13,10 +13,12 @@ def _np(t):
 def test_alone_equals_in_mix(fea, scenes_mod):
     import torch
     from fuzzy_ellipses_amd.serve import Server, poisson_requests
-    scenes = scenes_mod.resident_scenes(n_scenes=4, n_lo=64, n_hi=600)
+    # N up to 2048: the batches mix 8-pixel-tile requests (N >= 1024) with 16-pixel ones
+    scenes = scenes_mod.resident_scenes(n_scenes=4, n_lo=64, n_hi=2048)
     reqs = poisson_requests(scenes, rate=0, n=24, seed=11, backward_frac=0.4,
                             resolutions=((64, 64), (96, 128)), burst=True)
     mix = Server(scenes, max_batch=8, slots=2)
+    assert {mix.tile_of(r) for r in reqs} == {8, 16}
     st = mix.run(reqs)
     assert st["requests"] == 24 and st["mean_batch"] > 1
     mixed = {r.rid: (_np(r.depth).copy(), _np(r.alpha).copy(),

@@ -68,6 +68,20 @@ def test_keys_shell_view(fea, scenes_mod, view):
     print(f"shell view {view}: phase-1 pairs {k1}, phase-2 pairs {k2}")
 
 
+@pytest.mark.parametrize("ts,H,W,f", [(16, 128, 128, 120.0), (16, 240, 320, 300.0), (8, 128, 128, 120.0)])
+def test_keys_chunked_membership(fea, scenes_mod, ts, H, W, f):
+    """One small view of N=4096 (9, 35 and 16 membership groups): each group's Gaussians are
+    shared out over several blocks, whose partial counts, flags and bounds a finishing kernel
+    merges, and the phase-2 lower bounds of tiles without a phase-1 threshold come from a
+    chunked pre-pass.  The lists still equal the restatement."""
+    rng = np.random.default_rng(31)
+    means, prec, w = scenes_mod.random_gaussians(rng, 4096)
+    cam = scenes_mod.make_camera(3.0 * scenes_mod.random_dir(rng), H, W, f)
+    scene = scenes_mod.Scene(means, prec, w, [cam])
+    k1, k2 = _check(fea, scene, cam, ts)
+    print(f"chunked membership ts={ts} {H}x{W}: phase-1 pairs {k1}, phase-2 pairs {k2}")
+
+
 def test_dispatch_order_all_shell_views(fea, scenes_mod):
     """With 16 views in one call the orders run over every view's tiles."""
     import torch

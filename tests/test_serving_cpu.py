@@ -132,3 +132,12 @@ def test_planner_sizes_what_requests_are_charged(fea):
                            ctypes.byref(nbytes)) == 0
         # the planner's worst-case lists alone are 4 bytes per (tile, Gaussian)
         assert nbytes.value >= 4 * n * 15 * 20
+
+
+def test_tile_policy_depends_on_the_request_alone(fea):
+    """Requests with no tile forced: 8-pixel tiles for forward-only views of up to 320x240
+    pixels of scenes of 1024 Gaussians or more, 16-pixel tiles otherwise."""
+    from fuzzy_ellipses_amd.serve import tile_for
+    assert tile_for(4096, 128, 128) == 8 and tile_for(1024, 240, 320) == 8
+    assert tile_for(4096, 480, 640) == 16 and tile_for(1023, 128, 128) == 16 and tile_for(4096, 241, 320) == 16
+    assert tile_for(4096, 128, 128, backward=True) == 16
