@@ -47,6 +47,7 @@ struct Layout {
   int has_rays = 0;
   int ts = 16, gb = 64, mode = FE_MODE_CULLED, save = 0, reuse = 0;
   int feat_chunks = 0;   // blocks per tile of the feature kernel (0: no view wants features)
+  int split_views = 0;   // views whose tile lists the blend splits
   int64_t total_stiles = 0;
   int64_t total_rec = 0, total_tiles = 0, total_mask = 0, total_pix = 0, list_cap = 0;
   size_t o_header = 0, o_mask2 = 0, o_jobs = 0, o_counters = 0, o_flags = 0, o_rec = 0, o_rec2 = 0, o_tnp = 0,
@@ -186,6 +187,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
       jb.pix_base = pix;
     }
     max_n = std::max(max_n, sc.n);
+    if (view_splits_lists(o.tile, (int64_t)tx * ty)) ++L->split_views;
     rec += sc.n;
     tiles += (int64_t)tx * ty;
     stiles += (int64_t)((tx + sfv - 1) / sfv) * ((ty + sfv - 1) / sfv);
@@ -414,7 +416,7 @@ static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_s
   if (L.mode == FE_MODE_DENSE) {
     FE_HIP(timed(s, FE_K_BLEND_DENSE, [&] {
       return launch_blend(s, L.ts, L.gb, 0, L.total_tiles, dj, L.n_jobs, rec, rec2, nullptr, nullptr, nullptr,
-                          nullptr, state, L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
+                          nullptr, state, L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr, L.split_views);
     }, may_time));
     if (L.feat_chunks > 0)
       FE_HIP(launch_feat(s, false, L.ts, L.total_tiles, L.feat_chunks, dj, nullptr, L.n_jobs, rec, rec2, nullptr,
@@ -444,7 +446,7 @@ static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_s
       s, FE_K_WRITE1, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask, off1, list); }, may_time));
   FE_HIP(timed(s, FE_K_BLEND1, [&] {
     return launch_blend(s, L.ts, L.gb, 1, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
-                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save, tord);
+                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save, tord, L.split_views);
   }, may_time));
   // phase 2
   FE_HIP(timed(s, FE_K_MASK2, [&] {
@@ -460,7 +462,7 @@ static fe_status forward_impl(hipStream_t s, const fe_scene* scenes, int32_t n_s
       s, FE_K_WRITE2, [&] { return launch_tile_write(s, dj, L.n_jobs, L.total_tiles, mask2, off2, list); }, may_time));
   FE_HIP(timed(s, FE_K_BLEND2, [&] {
     return launch_blend(s, L.ts, L.gb, 2, L.total_tiles, dj, L.n_jobs, rec, rec2, tnp, off1, off2, list, state,
-                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr);
+                        L.total_pix, lmin, seed, ubm, cnt, flags, L.save, nullptr, L.split_views);
   }, may_time));
   if (L.feat_chunks > 0)
     FE_HIP(launch_feat(s, false, L.ts, L.total_tiles, L.feat_chunks, dj, nullptr, L.n_jobs, rec, rec2, off1, off2,

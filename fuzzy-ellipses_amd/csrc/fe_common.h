@@ -160,6 +160,12 @@ constexpr int kCountSlots = 64;
 // merges them and finishes the group's tiles; after them, one word per (block, tile) holds a
 // phase-2 lower bound for tiles that phase 1 left without a threshold.
 constexpr int kMaskChunkBlocks = 1024;
+// The blend splits the tile lists of views of fewer 16-pixel tiles than this (8-pixel tiles:
+// every view; 32-pixel tiles: none).
+constexpr int kSplitMinTiles = 1024;
+__host__ __device__ inline bool view_splits_lists(int ts, int64_t n_tiles) {
+  return ts == 8 || (ts == 16 && n_tiles < kSplitMinTiles);
+}
 constexpr int kMaskChunkGroups = kMaskChunkBlocks / 2;
 constexpr int kMaskMaxSub = 16;   // tiles per group at most (4x4 8-pixel tiles)
 struct Counters {

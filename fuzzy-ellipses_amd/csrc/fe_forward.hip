@@ -1117,16 +1117,20 @@ struct BlendArgs {
 // Pixels per thread of the blend: two for 16- and 32-pixel tiles (each record read from LDS
 // serves two pixels; with one, three 16-byte LDS broadcasts per pair bound the kernel), one
 // for 8-pixel tiles (whose 64 pixels fill one wave).
-// A tile's list of L records is split KS ways: virtual split v blends the contiguous range
-// [v h, min(L, (v + 1) h)) of it (h = L/KS rounded up to a multiple of SUB) from its own empty
-// state, and the KS states are merged at the end (max, then S and Z rescaled to it; E and Zall
-// added, NaN flags or-ed).  The arithmetic depends on KS alone, which is fixed per tile size and
-// phase, so a tile's result does not depend on the batch around it.  How the splits run is
-// chosen per launch (SPLIT): side by side on KS groups of NT threads, each round staging R =
-// GB/KS records of every range (a tile then runs on KS times the waves, for grids whose time is
-// set by their longest tiles' latency), or one after another on one group in rounds of GB
-// records, for a grid that fills the GPU many times over (fewer instructions per pair and
-// 2-wave blocks, which wait less at the round barriers).
+// The list of L records of a tile of a small view is split KS ways: split v blends the
+// contiguous range [v h, min(L, (v + 1) h)) of it (h = L/KS rounded up to a multiple of SUB)
+// from its own empty state, and the KS states are merged at the end (max, then S and Z
+// rescaled to it; E and Zall added, NaN flags or-ed).  Views of kSplitMinTiles 16-pixel tiles
+// or more are not split (view_splits_lists): their grids fill the GPU by themselves in a batch, where the split's
+// instructions cost (the 16-view 480x640 shell: phase-1 blend 0.87-0.89 ms unsplit against
+// 0.92-0.94 split), while it halves a small view's latency (one 128x128 view, N=4096:
+// 0.74 -> 0.35 ms).  Whether a tile is split depends on its view alone, so a view's result does
+// not depend on the batch around it.  How a launch runs (MODE): 0 when no view of the batch is
+// split (the plain kernel), else 2, side by side on KS groups of NT threads, each round staging
+// R = GB/KS records of every range (a tile runs on KS times the waves, for grids whose time is
+// set by their longest tiles' latency), or 1, one after another on one group in rounds of GB
+// records, for a grid that fills the GPU many times over (2-wave blocks, which wait less at the
+// round barriers).  An unsplit view's tiles go through every mode with the same operations.
 template <int TS, int PHASE>
 struct BlendShape {
   static constexpr int PPT = (TS >= 16) ? 2 : 1;
@@ -1137,29 +1141,32 @@ struct BlendShape {
   // (its blend is bound by loading and storing the pixel state), so it is not split, nor are
   // 32-pixel tiles (8 waves already).
   static constexpr int KS = (PHASE == 2 || NT >= 256) ? 1 : 256 / NT;
-  static constexpr int nb(bool split) { return NT * (split ? KS : 1); }   // threads per block
+  static constexpr int nb(int mode) { return NT * (mode == 2 ? KS : 1); }   // threads per block
 };
 
 // Waves per SIMD the blend of 16-pixel tiles is compiled for.  The phase-2 blend is held to 128
 // VGPRs (4 waves instead of 3): its tiles are short and its time is mostly loading and storing
 // the per-pixel state, which more resident waves hide better (0.112 -> 0.101 ms).  Phase 1 is
-// held to 7, except where LDS caps it first: the 2-wave blocks of larger LDS batches (staging
-// plus the merge buffer: 14 KB for GB = 128 and 22 KB for 256) fit 6 and 4 waves per SIMD.
-template <int TS, int GB, int PHASE, bool SPLIT>
+// held to 7, except where LDS caps it first: the 2-wave blocks of larger LDS batches fit 6 or 4
+// waves per SIMD (staging, plus the merge buffer in mode 1: 8 and 16 KB for GB = 128 and 256
+// unsplit, 14 and 22 KB in mode 1).
+template <int TS, int GB, int PHASE, int MODE>
 constexpr int blend_waves() {
   if (TS != 16) return 1;
   if (PHASE == 2) return 4;
   if (PHASE == 0) return 1;
-  return (SPLIT || GB <= 64) ? 7 : (GB == 128 ? 6 : 4);
+  if (MODE == 2 || GB <= 64) return 7;
+  if (MODE == 0) return GB == 128 ? 7 : 4;
+  return GB == 128 ? 6 : 4;
 }
 
-template <int TS, int GB, int PHASE, bool SPLIT>
-__global__ __launch_bounds__((BlendShape<TS, PHASE>::nb(SPLIT)), (blend_waves<TS, GB, PHASE, SPLIT>()))
+template <int TS, int GB, int PHASE, int MODE>
+__global__ __launch_bounds__((BlendShape<TS, PHASE>::nb(MODE)), (blend_waves<TS, GB, PHASE, MODE>()))
 void k_blend(BlendArgs A) {
   constexpr int PPT = BlendShape<TS, PHASE>::PPT;
   constexpr int NT = BlendShape<TS, PHASE>::NT;
-  constexpr int KS = BlendShape<TS, PHASE>::KS;
-  constexpr int KP = SPLIT ? KS : 1;     // groups side by side
+  constexpr int KS = (MODE == 0) ? 1 : BlendShape<TS, PHASE>::KS;   // splits of a split view
+  constexpr int KP = (MODE == 2) ? KS : 1;     // groups side by side
   constexpr int NB = NT * KP;
   // records per online-max update: 2 pairs per thread in the first pass, 4 in the others.
   // Fewer pairs in flight need fewer registers (70 and 88 VGPRs for 16-pixel tiles instead of
@@ -1172,7 +1179,7 @@ void k_blend(BlendArgs A) {
   static_assert((R & (R - 1)) == 0, "slices of a power-of-two size");
   __shared__ float4 lds[GB * 4];
   __shared__ float sh_red[NB / 64 + 1];
-  __shared__ float mrg[(KS > 1 ? KS - 1 : 1) * 6][NT * PPT];   // splits 1..KS-1: M S Z E Zall nan per pixel
+  __shared__ float mrg[(KS > 1 ? KS - 1 : 1) * 6][KS > 1 ? NT * PPT : 1];   // splits 1..KS-1: M S Z E Zall nan per pixel
   const int split = (KP > 1) ? __builtin_amdgcn_readfirstlane((int)threadIdx.x / NT) : 0;   // group (wave-uniform)
   const int ti = (int)threadIdx.x - split * NT;    // its pixel thread within the group
 
@@ -1240,35 +1247,55 @@ void k_blend(BlendArgs A) {
   const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
   // (a tile lists each record of its view at most once: its list length fits 32 bits)
   const int L = (int)(end - start);
-  const int h = ((L + KS - 1) / KS + SUB - 1) / SUB * SUB;   // records per split (the last may hold fewer)
+  const int ks = (KS > 1 && view_splits_lists(TS, jb.n_tiles)) ? KS : 1;   // this view's splits
+  const int h = ((L + ks - 1) / ks + SUB - 1) / SUB * SUB;   // records per split (the last may hold fewer)
   auto range_len = [&](int v) { return min(L, (v + 1) * h) - min(L, v * h); };
   // stage this round's records (P float4 each) of the groups' ranges into their LDS slots,
   // padded with null records to the slot's last SUB group
-  auto stage = [&](auto pc, int v, int o) {
+  auto null_part = [](auto pc, int part) {
     constexpr int P = decltype(pc)::value;
-    for (int i = threadIdx.x; i < KP * R * P; i += NB) {
-      const int k = i / P, part = i - P * k;
-      const int g = (KP > 1) ? k / R : 0, kk = k - g * R;   // group slot, record within it
-      const int vg = (KP > 1) ? g : v;
-      const int cg = min(R, range_len(vg) - o);   // the slot's records this round (may be <= 0)
-      if (kk >= (cg + SUB - 1) / SUB * SUB) continue;
-      float4 val;
-      if (kk < cg) {
-        const int li = vg * h + o + kk;
-        const int64_t gid = (PHASE == 0) ? (int64_t)li : (int64_t)A.list[start + li];
-        val = (P == 3) ? rec24[gid * 4 + part] : rec4[gid * 4 + part];
-      } else if (P == 3) {
-        // null record: X = 0, Y1 = 1, wL = -inf -> contributes nothing
-        val = (part == 1) ? make_float4(0.f, 1.f, 0.f, 0.f)
-            : (part == 2) ? make_float4(0.f, -INFINITY, 0.f, 0.f)
-                          : make_float4(0.f, 0.f, 0.f, 0.f);
-      } else {
-        // null record: u = (1,0,0), q = 0, w = -inf -> contributes nothing
-        val = (part == 1) ? make_float4(0.f, 0.f, 1.f, 0.f)
-            : (part == 3) ? make_float4(-INFINITY, -INFINITY, 0.f, 0.f)
-                          : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (P == 3)   // null blend record: X = 0, Y1 = 1, wL = -inf -> contributes nothing
+      return (part == 1) ? make_float4(0.f, 1.f, 0.f, 0.f)
+           : (part == 2) ? make_float4(0.f, -INFINITY, 0.f, 0.f)
+                         : make_float4(0.f, 0.f, 0.f, 0.f);
+    // null record: u = (1,0,0), q = 0, w = -inf -> contributes nothing
+    return (part == 1) ? make_float4(0.f, 0.f, 1.f, 0.f)
+         : (part == 3) ? make_float4(-INFINITY, -INFINITY, 0.f, 0.f)
+                       : make_float4(0.f, 0.f, 0.f, 0.f);
+  };
+  auto stage = [&](auto pc, int v, int o, int len) {
+    constexpr int P = decltype(pc)::value;
+    if constexpr (KP == 1) {
+      // one group: records [o, o + cnt) of range v, padded to the last SUB group
+      const int cnt = min(R, len - o), nbr = (cnt + SUB - 1) / SUB * SUB;
+      const int li0 = v * h + o;
+      for (int i = threadIdx.x; i < nbr * P; i += NB) {
+        const int k = i / P, part = i - P * k;
+        float4 val;
+        if (k < cnt) {
+          const int64_t gid = (PHASE == 0) ? (int64_t)(li0 + k) : (int64_t)A.list[start + li0 + k];
+          val = (P == 3) ? rec24[gid * 4 + part] : rec4[gid * 4 + part];
+        } else {
+          val = null_part(pc, part);
+        }
+        lds[i] = val;
       }
-      lds[i] = val;
+    } else {
+      for (int i = threadIdx.x; i < KP * R * P; i += NB) {
+        const int k = i / P, part = i - P * k;
+        const int g = k / R, kk = k - g * R;   // group slot, record within it
+        const int cg = min(R, range_len(g) - o);   // the slot's records this round (may be <= 0)
+        if (kk >= (cg + SUB - 1) / SUB * SUB) continue;
+        float4 val;
+        if (kk < cg) {
+          const int li = g * h + o + kk;
+          const int64_t gid = (PHASE == 0) ? (int64_t)li : (int64_t)A.list[start + li];
+          val = (P == 3) ? rec24[gid * 4 + part] : rec4[gid * 4 + part];
+        } else {
+          val = null_part(pc, part);
+        }
+        lds[i] = val;
+      }
     }
   };
   const int parts = r2 ? 3 : 4;   // float4 per record
@@ -1280,8 +1307,8 @@ void k_blend(BlendArgs A) {
     const int len = range_len(v);
     const int span = (KP > 1) ? range_len(0) : len;   // rounds of the block (split 0 is the longest)
     for (int o = 0; o < span; o += R) {
-      if (r2) stage(std::integral_constant<int, 3>{}, v, o);
-      else stage(std::integral_constant<int, 4>{}, v, o);
+      if (r2) stage(std::integral_constant<int, 3>{}, v, o, len);
+      else stage(std::integral_constant<int, 4>{}, v, o, len);
       __syncthreads();
       const int cnt = max(0, min(R, len - o));
       const int sq = (cnt + SUB - 1) / SUB * SUB;
@@ -1305,7 +1332,7 @@ void k_blend(BlendArgs A) {
     }
     // splits 1..KS-1 leave their states for the merge (one after another: in this thread's own
     // entries, which it reads back itself, then it starts the next split empty)
-    if (KS > 1 && v > 0) {
+    if (KS > 1 && ks > 1 && v > 0) {
 #pragma unroll
       for (int q = 0; q < PPT; ++q) {
         const int pi = q * NT + ti;   // consecutive lanes, consecutive words
@@ -1325,7 +1352,7 @@ void k_blend(BlendArgs A) {
 
   // merge the splits' states into split 0: the maximum, S and Z rescaled to it, E, Zall added
   if (KP > 1) __syncthreads();
-  if (KS > 1 && split == 0) {
+  if (KS > 1 && ks > 1 && split == 0) {
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
       const int pi = q * NT + ti;
@@ -1559,7 +1586,7 @@ hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t
   return hipGetLastError();
 }
 
-// Where a split tile list runs (the same arithmetic either way).  Side by side halves a block's
+// Where a split tile list runs (the same operations either way).  Side by side halves a block's
 // latency, which pays wherever the grid's last blocks set the time: measured phase-1 blends of
 // N=4096 views, side by side against one after another, 16-pixel tiles: one 128x128 view 0.35
 // against 0.74 ms, 64 such views 0.41 against 0.74, 8 480x640 views 0.43 against 0.44, and the
@@ -1569,37 +1596,42 @@ hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t
 template <int TS, int PHASE>
 static bool blend_side_by_side(int64_t total_tiles) {
   if (TS == 8) return true;
-  constexpr int waves = BlendShape<TS, PHASE>::nb(true) / 64;
+  constexpr int waves = BlendShape<TS, PHASE>::nb(2) / 64;
   return total_tiles * waves <= (int64_t)8 * device_cus() * 28;   // 28 waves per CU at once
 }
 
 template <int TS, int GB, int PHASE>
-static void launch_blend_p(hipStream_t s, int64_t total_tiles, const BlendArgs& a) {
+static void launch_blend_p(hipStream_t s, int64_t total_tiles, bool any_split, const BlendArgs& a) {
   const dim3 grid((unsigned)total_tiles);
   if constexpr (BlendShape<TS, PHASE>::KS > 1) {
-    if (blend_side_by_side<TS, PHASE>(total_tiles)) {
-      hipLaunchKernelGGL((k_blend<TS, GB, PHASE, true>), grid, dim3(BlendShape<TS, PHASE>::nb(true)), 0, s, a);
+    if (any_split) {
+      if (blend_side_by_side<TS, PHASE>(total_tiles))
+        hipLaunchKernelGGL((k_blend<TS, GB, PHASE, 2>), grid, dim3(BlendShape<TS, PHASE>::nb(2)), 0, s, a);
+      else
+        hipLaunchKernelGGL((k_blend<TS, GB, PHASE, 1>), grid, dim3(BlendShape<TS, PHASE>::nb(1)), 0, s, a);
       return;
     }
   }
-  hipLaunchKernelGGL((k_blend<TS, GB, PHASE, false>), grid, dim3(BlendShape<TS, PHASE>::nb(false)), 0, s, a);
+  hipLaunchKernelGGL((k_blend<TS, GB, PHASE, 0>), grid, dim3(BlendShape<TS, PHASE>::nb(0)), 0, s, a);
 }
 
 template <int TS, int GB>
-static hipError_t launch_blend_tg(hipStream_t s, int phase, int64_t total_tiles, const BlendArgs& a) {
-  if (phase == 0) launch_blend_p<TS, GB, 0>(s, total_tiles, a);
-  else if (phase == 1) launch_blend_p<TS, GB, 1>(s, total_tiles, a);
-  else launch_blend_p<TS, GB, 2>(s, total_tiles, a);
+static hipError_t launch_blend_tg(hipStream_t s, int phase, int64_t total_tiles, bool any_split,
+                                  const BlendArgs& a) {
+  if (phase == 0) launch_blend_p<TS, GB, 0>(s, total_tiles, any_split, a);
+  else if (phase == 1) launch_blend_p<TS, GB, 1>(s, total_tiles, any_split, a);
+  else launch_blend_p<TS, GB, 2>(s, total_tiles, any_split, a);
   return hipGetLastError();
 }
 
 template <int TS>
-static hipError_t launch_blend_t(hipStream_t s, int gb, int phase, int64_t total_tiles, const BlendArgs& a) {
+static hipError_t launch_blend_t(hipStream_t s, int gb, int phase, int64_t total_tiles, bool any_split,
+                                 const BlendArgs& a) {
   switch (gb) {
-    case 32: return launch_blend_tg<TS, 32>(s, phase, total_tiles, a);
-    case 64: return launch_blend_tg<TS, 64>(s, phase, total_tiles, a);
-    case 128: return launch_blend_tg<TS, 128>(s, phase, total_tiles, a);
-    case 256: return launch_blend_tg<TS, 256>(s, phase, total_tiles, a);
+    case 32: return launch_blend_tg<TS, 32>(s, phase, total_tiles, any_split, a);
+    case 64: return launch_blend_tg<TS, 64>(s, phase, total_tiles, any_split, a);
+    case 128: return launch_blend_tg<TS, 128>(s, phase, total_tiles, any_split, a);
+    case 256: return launch_blend_tg<TS, 256>(s, phase, total_tiles, any_split, a);
   }
   return hipErrorInvalidValue;
 }
@@ -1609,13 +1641,13 @@ hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_
                         const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
                         float* tile_lmin, float* tile_seed, const float* tile_ubm, Counters* cnt,
-                        const int32_t* job_flags, int save, const int32_t* tord) {
+                        const int32_t* job_flags, int save, const int32_t* tord, int split_views) {
   BlendArgs a{jobs, n_jobs, rec, rec2, tile_np, off1, off2, list, st, total_pix, tile_lmin, tile_seed, tile_ubm, cnt,
               job_flags, save, tord};
   switch (ts) {
-    case 8: return launch_blend_t<8>(s, gb, phase, total_tiles, a);
-    case 16: return launch_blend_t<16>(s, gb, phase, total_tiles, a);
-    case 32: return launch_blend_t<32>(s, gb, phase, total_tiles, a);
+    case 8: return launch_blend_t<8>(s, gb, phase, total_tiles, split_views > 0, a);
+    case 16: return launch_blend_t<16>(s, gb, phase, total_tiles, split_views > 0, a);
+    case 32: return launch_blend_t<32>(s, gb, phase, total_tiles, split_views > 0, a);
   }
   return hipErrorInvalidValue;
 }

@@ -39,12 +39,14 @@ hipError_t launch_scan(hipStream_t s, const uint32_t* cnt, int64_t n, unsigned l
 hipError_t launch_tile_write(hipStream_t s, const Job* jobs, int n_jobs, int64_t total_tiles,
                              const unsigned long long* mask, const unsigned long long* off,
                              uint32_t* list);
+// split_views: views of the batch whose tile lists the blend splits (view_splits_lists)
 hipError_t launch_blend(hipStream_t s, int ts, int gb, int phase, int64_t total_tiles, const Job* jobs,
                         int n_jobs, const float* rec, const float* rec2, const uint32_t* tile_np,
                         const unsigned long long* off1,
                         const unsigned long long* off2, const uint32_t* list, float* st, int64_t total_pix,
                         float* tile_lmin, float* tile_seed, const float* tile_ubm, Counters* cnt,
-                        const int32_t* job_flags, int save, const int32_t* tord);
+                        const int32_t* job_flags, int save, const int32_t* tord,
+                        int split_views);
 
 // With feat_chunks > 0 the feature backward (k_feat_bwd) runs between the blend and the
 // per-record chain rule.
