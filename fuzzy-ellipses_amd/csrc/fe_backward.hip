@@ -191,7 +191,8 @@ struct BwdArgs {
   float* gsum;          // the 11 blend-record sums per (Gaussian, view) record, one plane each
   int64_t total_rec;    // plane stride
   int dense;
-  const int32_t* tord;  // dispatch order (block b takes tile tord[b]), or null
+  const int32_t* tord;  // dispatch order (blocks of tile tord[b] are b kb .. b kb + kb - 1), or null
+  int kb;               // blocks per tile (each takes every kb-th chunk of 64 Gaussians)
 };
 
 // Sums per record (kGsumPlanes = 11).  Planes, not records of 11 floats: device-scope float
@@ -381,9 +382,10 @@ __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* p
   G[9] = S9; G[10] = S10;
 }
 
-// One block (4 waves) per tile; lane = one Gaussian of the tile's lists (phase 1 then
-// phase 2, or all Gaussians in dense mode).  The tile's per-pixel terms (x', y', M, D and
-// the cotangent factors) are staged once in LDS and read as wave-uniform broadcasts.
+// One block (4 waves) per tile, or kb on small grids (each taking every kb-th chunk of 64
+// Gaussians); lane = one Gaussian of the tile's lists (phase 1 then phase 2, or all Gaussians
+// in dense mode).  The tile's per-pixel terms (x', y', M, D and the cotangent factors) are
+// staged once per block in LDS and read as wave-uniform broadcasts.
 //  * Views with blend records: the Gaussians go in chunks of 64, one lane each, and every
 //    wave takes the same chunk over its own quarter of the tile's pixels, so the 4 waves
 //    carry equal work whatever the list length.  Each lane keeps 11 sums in registers; the 4
@@ -401,8 +403,10 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   __shared__ float2 pxc[NPIX];   // 1e-20 cD, cX (maximum-logit term)
   __shared__ float red[4][11][64];
 
-  // one block per tile
-  const int64_t gt = A.tord ? (int64_t)A.tord[blockIdx.x] : xcd_tile(blockIdx.x, gridDim.x);
+  // kb blocks per tile (kb > 1 on small grids), next to each other in the dispatch order
+  const int64_t kb = A.kb, cy = blockIdx.x % A.kb;
+  const int64_t bt = blockIdx.x / A.kb, ntiles = gridDim.x / A.kb;
+  const int64_t gt = A.tord ? (int64_t)A.tord[bt] : xcd_tile(bt, ntiles);
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
   const Job& jb = A.jobs[ji];
   const BwdJob& bj = A.bjobs[ji];
@@ -418,7 +422,9 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
     s2 = (int64_t)A.off2[gt]; n2 = (int64_t)A.off2[gt + 1] - s2;
   }
   const int64_t total = n1 + n2;
-  if (total == 0) return;
+  // the tile's chunks of 64 Gaussians are shared out over its kb blocks (chunk c to block
+  // c mod kb; the sums go out by atomics either way)
+  if (total <= 64 * cy) return;   // no chunk for this block (block-uniform)
   const float ncnt = (float)total;
 
   const int32_t jflags = A.job_flags[ji];
@@ -483,10 +489,10 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
         qc = rec24[gid_n * 4 + 2];
       }
     };
-    fetch(0);
+    fetch(64 * cy);
     asm volatile("" ::"v"(qa.x), "v"(qa.y), "v"(qa.z), "v"(qa.w), "v"(qb.x), "v"(qb.y), "v"(qb.z), "v"(qb.w),
                  "v"(qc.x), "v"(qc.y), "v"(qc.z), "v"(qc.w));
-    for (int64_t c0 = 0; c0 < total; c0 += 64) {
+    for (int64_t c0 = 64 * cy; c0 < total; c0 += 64 * kb) {
       const int sp = split_of(c0);   // block-uniform
       const bool act = c0 + lane_of(sp) < total;
       const int64_t gid = gid_n;
@@ -538,7 +544,7 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
 #pragma unroll
       for (int k = 0; k < 11; ++k) red[wave][k][lane] = G[k];
       __syncthreads();
-      if (c0 + 64 < total) fetch(c0 + 64);
+      if (c0 + 64 * kb < total) fetch(c0 + 64 * kb);
       // wave w adds components k = w, w+4, w+8 of the chunk's 64 Gaussians
       float v[3];
 #pragma unroll
@@ -564,7 +570,7 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   const bool safe = (jflags & kFlagDegenerate) != 0;   // the view has degenerate records
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
 
-  for (int64_t c0 = (int64_t)wave * 64; c0 < total; c0 += 256) {
+  for (int64_t c0 = (cy * 4 + wave) * 64; c0 < total; c0 += 256 * kb) {
     const int64_t kk = c0 + lane;
     if (kk >= total) break;
     int64_t gid;
@@ -627,8 +633,21 @@ namespace fe {
 
 template <int TS>
 static hipError_t bwd_t(hipStream_t s, int64_t total_tiles, const BwdArgs& a) {
-  hipLaunchKernelGGL((k_bwd_blend<TS>), dim3((unsigned)total_tiles), dim3(256), 0, s, a);
+  hipLaunchKernelGGL((k_bwd_blend<TS>), dim3((unsigned)(total_tiles * a.kb)), dim3(256), 0, s, a);
   return hipGetLastError();
+}
+
+// Blocks per tile of the backward blend.  A grid of few tiles leaves most of the GPU idle while
+// each block walks its tile's list chunk by chunk, so up to 8 blocks share a tile until the grid
+// has about 12 blocks per CU.  Lone views, N=4096: 128x128 (64 tiles, lists up to 3241
+// records) 445 -> 79 us, 240x320 362 -> 78 us, 480x640 333 -> 197 us (2 blocks per tile); with
+// about 4 blocks per CU the last two took 143 and 333 us.  Larger grids keep one block per tile:
+// a second block repeats the tile's per-pixel set-up (the 16-view 480x640 shell: 1.38 -> 1.48 ms
+// with two).
+static int bwd_blocks_per_tile(int64_t total_tiles) {
+  const int64_t target = (int64_t)12 * device_cus();
+  if (total_tiles <= 0 || 2 * total_tiles > target) return 1;
+  return (int)std::min<int64_t>(8, target / total_tiles);
 }
 
 hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Job* jobs, const void* bjobs,
@@ -641,7 +660,7 @@ hipError_t launch_bwd_blend(hipStream_t s, int ts, int64_t total_tiles, const Jo
     hipLaunchKernelGGL(k_fill_zero, dim3((unsigned)std::min<int64_t>((n4 + 255) / 256, 4096)), dim3(256), 0, s,
                        reinterpret_cast<float4*>(gsum), n4);
   BwdArgs a{jobs, reinterpret_cast<const BwdJob*>(bjobs), n_jobs, rec, rec2, tile_np, off1, off2, list, st,
-            total_pix, job_flags, gsum, total_rec, dense, tord};
+            total_pix, job_flags, gsum, total_rec, dense, tord, bwd_blocks_per_tile(total_tiles)};
   hipError_t e = hipErrorInvalidValue;
   switch (ts) {
     case 8: e = bwd_t<8>(s, total_tiles, a); break;

@@ -1480,7 +1480,7 @@ void k_blend(BlendArgs A) {
 namespace fe {
 
 // CUs of the current device (cached per device; 256 if the query fails)
-static int device_cus() {
+int device_cus() {
   static std::atomic<int> cache[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;

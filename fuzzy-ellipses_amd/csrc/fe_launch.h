@@ -31,6 +31,7 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
                             uint32_t* count, const float* tile_lmin, const float* tile_seed, float* tile_ubm,
                             uint32_t* tile_np, const int32_t* job_flags, Counters* cnt, uint32_t* mpart);
 int64_t scan_partial_count(int64_t n);
+int device_cus();   // CUs of the current device (cached per device; 256 if the query fails)
 // With hist and tord (both or neither), also writes the tiles' dispatch order to tord (a
 // counting sort over kTileBuckets buckets of the tile key; hist must start at zero).
 hipError_t launch_scan(hipStream_t s, const uint32_t* cnt, int64_t n, unsigned long long* partial,
