@@ -320,6 +320,47 @@ def workload_silhouette(fea, sc, dev, steps):
                                    "240x320, BCE + 0.1 L1, Adam lr 1e-2"}}
 
 
+def workload_lone(fea, sc, dev, steps):
+    """Latency of a lone request: one view per call, synchronised after each, N=4096 Gaussians
+    in the unit cube seen from radius 3, forward and forward+backward, at 128x128, 240x320 and
+    480x640 (f = W * 600/640), with 16-pixel tiles and with the server's tile rule (tile_for)."""
+    import numpy as np
+    import torch
+    from fuzzy_ellipses_amd.serve import tile_for
+    rng = np.random.default_rng(6)
+    means, prec, w = sc.random_gaussians(rng, 4096)
+    st = fea.SceneTensors.from_scene(sc.Scene(means, prec, w, []), dev)
+    gen = torch.Generator(device="cpu").manual_seed(6)
+    res = {}
+    for H, W in ((128, 128), (240, 320), (480, 640)):
+        cam = sc.make_camera(3.0 * sc.random_dir(rng), H, W, W * 600.0 / 640.0)
+        gd = torch.randn(H, W, generator=gen).to(dev)
+        ga = torch.randn(H, W, generator=gen).to(dev)
+        for bwd in (False, True):
+            for tile in sorted({16, tile_for(4096, H, W, bwd)}):
+                r = fea.Renderer(tile=tile, device=dev)
+
+                def call():
+                    r.forward([st], [(0, cam, 21.4, 2.66)], save_for_backward=bwd)
+                    if bwd:
+                        r.backward([gd], [ga])
+                for _ in range(3):
+                    call()
+                torch.cuda.synchronize()
+                ts = []
+                for _ in range(steps):
+                    t0 = time.perf_counter()
+                    call()
+                    torch.cuda.synchronize()
+                    ts.append(time.perf_counter() - t0)
+                res[f"{H}x{W}_{'fwd_bwd' if bwd else 'fwd'}_tile{tile}_ms"] = round(1e3 * float(np.median(ts)), 4)
+    return {"metric": "lone-request latency, median ms per synchronised call (N=4096)",
+            "value": res["128x128_fwd_bwd_tile16_ms"], "unit": "ms", "n_gpus": 1, "steps": steps,
+            "higher_is_better": False, "dtype": "f32", "data": "synthetic", "latency_ms": res,
+            "config": {"workload": "one view per call of N=4096 Gaussians in the unit cube from radius 3; "
+                                   "value = 128x128 forward+backward, 16-pixel tiles"}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -329,7 +370,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64)
     ap.add_argument("--mode", default="culled")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--workload", default="shell", choices=["shell", "batched", "serving", "silhouette", "gmm"])
+    ap.add_argument("--workload", default="shell", choices=["shell", "batched", "serving", "silhouette", "gmm", "lone"])
     ap.add_argument("--rates", default="50,200,800")
     ap.add_argument("--requests", default="300,600,1200")
     ap.add_argument("--no-serving", action="store_true")
@@ -372,6 +413,9 @@ def main():
         return
     if args.workload == "silhouette":
         print(json.dumps(workload_silhouette(fea, sc, dev, max(args.steps, 200))), flush=True)
+        return
+    if args.workload == "lone":
+        print(json.dumps(workload_lone(fea, sc, dev, max(args.steps, 20))), flush=True)
         return
     scene = sc.shell_scene()
     cams = scene.cameras
