@@ -691,7 +691,17 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
 }
 
 // Merges the partials of a group's chunks (k_tile_mask with chunks > 1) and finishes its
-// tiles: one wave per group.
+// tiles: one wave per group.  The lanes load every (tile, chunk) partial at once and combine
+// them in LDS (count: sum, flag: or, bound: max through an order-preserving integer key; NaN
+// bounds are dropped, as fmaxf drops them), then the wave finishes the tiles one by one.
+__device__ __forceinline__ int float_order_key(float f) {
+  const int b = __float_as_int(f);
+  return b >= 0 ? b : (b ^ 0x7FFFFFFF);
+}
+__device__ __forceinline__ float float_from_order_key(int k) {
+  return __int_as_float(k >= 0 ? k : (k ^ 0x7FFFFFFF));
+}
+
 template <int PHASE>
 __global__ __launch_bounds__(64) void k_mask_finish(const Job* __restrict__ jobs, int n_jobs, int chunks,
                                                     const uint32_t* __restrict__ mpart,
@@ -700,6 +710,8 @@ __global__ __launch_bounds__(64) void k_mask_finish(const Job* __restrict__ jobs
                                                     uint32_t* __restrict__ count, const float* __restrict__ tile_seed,
                                                     float* __restrict__ tile_ubm, uint32_t* __restrict__ tile_np,
                                                     Counters* __restrict__ cnt_out) {
+  __shared__ uint32_t s_cnt[kMaskMaxSub], s_np[kMaskMaxSub];
+  __shared__ int s_ubm[kMaskMaxSub];
   const int64_t gs = blockIdx.x;
   const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
   const Job& jb = jobs[ji];
@@ -708,19 +720,30 @@ __global__ __launch_bounds__(64) void k_mask_finish(const Job* __restrict__ jobs
   const int sx = st % jb.stiles_x, sy = st / jb.stiles_x;
   const int tx0 = sx * sf, ty0 = sy * sf;
   const int nsx = min(sf, jb.tiles_x - tx0), nsy = min(sf, jb.tiles_y - ty0);
+  const int nsub = nsx * nsy;
   const int lane = threadIdx.x;
-  for (int k = 0; k < nsx * nsy; ++k) {
-    uint32_t total = 0, np = 0;
-    float u = -INFINITY;
-    for (int q = 0; q < chunks; ++q) {
-      const uint32_t* mp = mpart + ((int64_t)gs * chunks + q) * (4 * kMaskMaxSub) + 4 * k;
-      total += mp[0];
-      np |= mp[1];
-      u = fmaxf(u, __uint_as_float(mp[2]));
-    }
-    const float lm = __uint_as_float(mpart[(int64_t)gs * chunks * (4 * kMaskMaxSub) + 4 * k + 3]);
+  if (lane < kMaskMaxSub) {
+    s_cnt[lane] = 0;
+    s_np[lane] = 0;
+    s_ubm[lane] = float_order_key(-INFINITY);
+  }
+  __syncthreads();
+  const uint32_t* base = mpart + gs * chunks * (4 * kMaskMaxSub);
+  for (int f = lane; f < nsub * chunks; f += 64) {
+    const int q = f / nsub, k = f - q * nsub;
+    const uint32_t* mp = base + (int64_t)q * (4 * kMaskMaxSub) + 4 * k;
+    const uint32_t c = mp[0], np = mp[1];
+    const float u = __uint_as_float(mp[2]);
+    if (c) atomicAdd(&s_cnt[k], c);
+    if (np) atomicOr(&s_np[k], np);
+    if (u == u) atomicMax(&s_ubm[k], float_order_key(u));
+  }
+  __syncthreads();
+  for (int k = 0; k < nsub; ++k) {
+    const float lm = __uint_as_float(base[4 * k + 3]);   // every chunk holds the same threshold
     const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
-    finish_tile<PHASE>(jb, tile, total, np, u, lm, lane, mask1, mask2, count, tile_seed, tile_ubm, tile_np, cnt_out);
+    finish_tile<PHASE>(jb, tile, s_cnt[k], s_np[k], float_from_order_key(s_ubm[k]), lm, lane, mask1, mask2, count,
+                       tile_seed, tile_ubm, tile_np, cnt_out);
   }
 }
 
