@@ -1089,18 +1089,22 @@ __device__ __forceinline__ void blend_batch_rec2(const float4* lds, int nbr, con
         Zall[q] += t;
         ls[q][s] = lt;
         tsv[q][s] = t;
-        mx[q] = fmaxf(mx[q], lt);
+        if (SUB > 1) mx[q] = fmaxf(mx[q], lt);
       }
     }
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
-      if (mx[q] > Mt[q]) {
-        const float Mn = b3 * mx[q];
+      // With one record per update the maximum rises exactly when the record's logit exceeds
+      // it (a NaN logit compares false, as fmaxf ignores it): one compare instead of a
+      // canonicalising fmaxf and a compare per pair.
+      const float mq = (SUB == 1) ? ls[q][0] : mx[q];
+      if (mq > Mt[q]) {
+        const float Mn = b3 * mq;
         const float sc = __builtin_amdgcn_exp2f(M[q] - Mn);
         S[q] *= sc;
         Z[q] *= sc;
         M[q] = Mn;
-        Mt[q] = mx[q];
+        Mt[q] = mq;
       }
     }
 #pragma unroll
@@ -1191,12 +1195,14 @@ void k_blend(BlendArgs A) {
   constexpr int KS = (MODE == 0) ? 1 : BlendShape<TS, PHASE>::KS;   // splits of a split view
   constexpr int KP = (MODE == 2) ? KS : 1;     // groups side by side
   constexpr int NB = NT * KP;
-  // records per online-max update: 2 pairs per thread in the first pass, 4 in the others.
-  // Fewer pairs in flight need fewer registers (70 and 88 VGPRs for 16-pixel tiles instead of
-  // 112 and 128), and the extra resident waves hide the LDS and transcendental latencies: with
-  // 8 pairs the first pass took 0.88 ms, with 2 it takes 0.82-0.83 ms, and the second pass
-  // 0.093 -> 0.087 ms with 4.
-  constexpr int SUB = (PHASE == 1) ? 2 / PPT : 4 / PPT;
+  // records per online-max update: 2 pairs per thread (one record for threads of two pixels).
+  // Fewer pairs in flight need fewer registers (70 and 88 VGPRs for 16-pixel tiles with 2 and
+  // 4 pairs instead of 112 and 128 with 8), and the extra resident waves hide the LDS and
+  // transcendental latencies (the first pass 6% faster with 2 pairs than with 8).  With one
+  // record per update the update is a compare per pixel, with no maximum to take: phase-1
+  // blend of the 16-view 480x640 shell 0.90-0.92 -> 0.83-0.84 ms (interleaved runs on one
+  // box); the second pass takes the same time with one record or two (0.085-0.087 ms).
+  constexpr int SUB = 2 / PPT;
   constexpr int R = GB / KP;             // records staged per group and round
   static_assert(GB % (KS * SUB) == 0, "a round's slices fit the LDS batch");
   static_assert((R & (R - 1)) == 0, "slices of a power-of-two size");
