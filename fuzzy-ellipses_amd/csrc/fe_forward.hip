@@ -1201,7 +1201,9 @@ void k_blend(BlendArgs A) {
   // transcendental latencies (the first pass 6% faster with 2 pairs than with 8).  With one
   // record per update the update is a compare per pixel, with no maximum to take: phase-1
   // blend of the 16-view 480x640 shell 0.90-0.92 -> 0.83-0.84 ms (interleaved runs on one
-  // box); the second pass takes the same time with one record or two (0.085-0.087 ms).
+  // box).  The second pass with one record per update instead of two: the same time on the
+  // shell (0.085-0.087 ms), the 256-scene batch 2% faster, a lone 128x128 view's second pass
+  // 15% slower (its call 3%).
   constexpr int SUB = 2 / PPT;
   constexpr int R = GB / KP;             // records staged per group and round
   static_assert(GB % (KS * SUB) == 0, "a round's slices fit the LDS batch");
