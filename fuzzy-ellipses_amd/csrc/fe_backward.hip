@@ -206,7 +206,7 @@ struct PixTerms {
 };
 
 __device__ __forceinline__ PixTerms pix_terms(const BwdArgs& A, const Job& jb, const BwdJob& bj, int64_t lp, int px,
-                                              int py, float ncnt) {
+                                              int py, float ncnt, bool rec2) {
   PixTerms t;
   t.xf = jb.ray_xy ? jb.ray_xy[2 * lp] : (float)px - jb.cx;
   t.yf = jb.ray_xy ? jb.ray_xy[2 * lp + 1] : (float)py - jb.cy;
@@ -232,6 +232,25 @@ __device__ __forceinline__ PixTerms pix_terms(const BwdArgs& A, const Job& jb, c
   t.B1 = -(t.cDb * D);
   if (cD == 0.0f || !(t.A1 == t.A1) || !(t.B1 == t.B1)) { t.A1 = 0.0f; t.A2 = 0.0f; t.B1 = 0.0f; }
   t.floor_px = !(S >= 1e-11f * ncnt);
+  // A pixel whose weight sum is exactly that of its maximum-logit pair (every other weight
+  // was rounded away in the forward's sums) has dD/dl = 0 for all its pairs in the forward's
+  // arithmetic: its D is that pair's t, and the others' weights are below float resolution.
+  // Computed as b3 cD s (t - D), the dominant pair's term would be a difference of two
+  // separately rounded values of size |D| (D = Z/S from the forward's sums, t the pair's
+  // own), an error of ~2^-23 |D| that the chain rule scales by dd2/dtheta, large at pixels
+  // far (in whitened units) from the Gaussian.  Such pixels keep only dL/dt = cD s and the
+  // alpha term.  The maximum pair's weight is recomputed with the forward's operations from
+  // the exact maximum Mt it saved.
+  if (rec2 && !t.floor_px && S > 0.0f) {
+    const float Mt = A.st[6 * A.total_pix + pix];
+    if (S == __builtin_amdgcn_exp2f(fmaf(jb.beta3, Mt, -M))) {
+      t.A1 = cD;
+      t.A2 = 0.0f;
+      t.cDb = 0.0f;
+      t.B1 = 0.0f;
+      t.cX = 0.0f;
+    }
+  }
   return t;
 }
 
@@ -285,30 +304,35 @@ __device__ __forceinline__ void bwd_pixels_rec2(const float4* pxa, const float4*
 }
 
 // Row form of the blend-record loop for pixel-grid tiles (row width wv), pixels [p0, p1).
-// pxa holds (x', x'^2, M, cE) and pxb (A1, A2, b3 cD, B1) with tile-local coordinates
-// x' = ix - hc, y' = iy - hc (hc = (TS-1)/2); pxc (1e-20 cD, cX).  The forms X, Y1, Y2 take
-// their y-part once per row.  The sums go per row and per tile-local moment:
+// pxa holds (x', x'^2, M, cE) and pxb (A1, A2, b3 cD, B1) with (x', y') the pixel's
+// coordinates relative to the centre of its 32x32-pixel block (form_coord); pxc (1e-20 cD,
+// cX).  The forms X, Y1, Y2 take their y-part once per row, with the record's constant terms
+// moved to the block's centre, by the forward blend's operations: every t*, d2 and weight s recomputed here is the
+// forward's value bit for bit.  (Where logits are large, at pixels far in whitened units
+// from every Gaussian, a logit rounded differently in the two passes would move its weight
+// by a few percent, 2^-24 |logit| in the exponent, which the forward's S, made from its own
+// weights, would no longer cancel; it also lets the argmax variant, on tiles where the
+// 1e-20 floor is visible, find the pair holding the maximum logit by equality.)  The sums
+// go per row and per tile-local moment:
 //   sum g_X p' from sum g_X x' and per-row sums of g_X,
 //   sum c Y_i p' = C k_i' with C = sum c p' p'^T (6 moments) and k_i' the form's local
-//   coefficients, so the per-pixel work does not depend on the two forms Y1, Y2,
+//   coefficients, so the per-pixel work does not depend on the two forms Y1, Y2
+//   (local coordinates keep the moments small: |x'|, |y'| <= 15.5),
 // and are returned for the global p = p' + (ox, oy, 0) in the layout of bwd_pixels_rec2.
-// The argmax variant (tiles where the 1e-20 floor is visible) evaluates X, Y1, Y2 at the
-// global coordinates with the forward's exact operations, so that the pair holding the
-// maximum logit is found by equality; elsewhere they use the tile-local form.
 // HALVES: the two half-waves take consecutive row blocks of the range (lanes 32-63 are
 // shifted down by roff rows) for the same 32 Gaussians; the caller adds the halves' sums.
 template <bool MASK, bool ARGMAX, bool HALVES>
 __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* pxb, const float2* pxc, int p0,
-                                              int p1, int roff, int wv, float hc, int px0, int py0, float cx,
+                                              int p1, int roff, int wv, int px0, int py0, float cx,
                                               float cy, const float* k, float wL, float ncG, float qq, float b3,
                                               float b2, float kap, float* G) {
   const float kx0 = k[0], kx1 = k[1], k10 = k[3], k11 = k[4], k20 = k[6], k21 = k[7];
-  // global coordinates of the local origin, and the forms' constants there
-  const float ox = ((float)px0 + hc) - cx, oy = ((float)py0 + hc) - cy;
+  // global coordinates of the block's centre, and the forms' constants there (the operations
+  // of localize_rec2, as the forward staged them)
+  const float ox = form_origin(px0, cx), oy = form_origin(py0, cy);
   const float kxl = fmaf(kx0, ox, fmaf(kx1, oy, k[2]));
   const float k1l = fmaf(k10, ox, fmaf(k11, oy, k[5]));
   const float k2l = fmaf(k20, ox, fmaf(k21, oy, k[8]));
-  const float kxr = ARGMAX ? k[2] : kxl, k1r = ARGMAX ? k[5] : k1l, k2r = ARGMAX ? k[8] : k2l;
   float SgXx = 0.f, SgXy = 0.f, SgX = 0.f;
   float Cxx = 0.f, Cxy = 0.f, Cyy = 0.f, Cx = 0.f, Cy = 0.f, C1 = 0.f;
   float S9 = 0.f, S10 = 0.f;
@@ -318,19 +342,17 @@ __device__ __forceinline__ void bwd_rows_rec2(const float4* pxa, const float4* p
   while (p < p1) {
     const int iy = p / wv;
     const int pe = min(p1, (iy + 1) * wv);
-    const float yl = (float)(iy + ro) - hc;
-    const float yr = ARGMAX ? (float)(py0 + iy + ro) - cy : yl;
-    const float XR = fmaf(kx1, yr, kxr), Y1R = fmaf(k11, yr, k1r), Y2R = fmaf(k21, yr, k2r);
+    const float yl = form_coord(py0 + iy + ro);
+    const float XR = fmaf(kx1, yl, kxl), Y1R = fmaf(k11, yl, k1l), Y2R = fmaf(k21, yl, k2l);
     float Rg = 0.f, Rcx = 0.f, Rc = 0.f;
     for (; p < pe; ++p) {
       const float4 pa = pxa[p + poff];
       const float4 pb = pxb[p + poff];
       const float xl = pa.x, xl2 = pa.y, M = pa.z, cE = pa.w;
       const float A1 = pb.x, A2 = pb.y, cDb = pb.z, B1 = pb.w;
-      const float xr = ARGMAX ? (float)(px0 + (p - iy * wv)) - cx : xl;
-      const float X = fmaf(kx0, xr, XR);
-      const float Y1 = fmaf(k10, xr, Y1R);
-      const float Y2 = fmaf(k20, xr, Y2R);
+      const float X = fmaf(kx0, xl, XR);
+      const float Y1 = fmaf(k10, xl, Y1R);
+      const float Y2 = fmaf(k20, xl, Y2R);
       const float W = fmaf(Y1, Y1, Y2 * Y2);
       const float rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
       const float t = X * rr;
@@ -430,16 +452,15 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
   const int32_t jflags = A.job_flags[ji];
   const bool rec2 = use_rec2(jb, jflags);
   const bool rows = rec2 && !jb.ray_xy;   // pixel-grid tile: row form, tile-local coordinates
-  const float hc = 0.5f * (float)(TS - 1);
   bool floor_px = false;
   for (int i = threadIdx.x; i < npx; i += 256) {
     const int ix = i % wv, iy = i / wv;
     const int px = tx * TS + ix, py = ty * TS + iy;
     const int64_t lp = (int64_t)py * jb.W + px;
-    const PixTerms P = pix_terms(A, jb, bj, lp, px, py, ncnt);
+    const PixTerms P = pix_terms(A, jb, bj, lp, px, py, ncnt, rec2);
     floor_px = floor_px || P.floor_px;
     if (rows) {
-      const float xl = (float)ix - hc;
+      const float xl = form_coord(tx * TS + ix);
       pxa[i] = make_float4(xl, xl * xl, P.M, P.cE);
       pxb[i] = make_float4(P.A1, P.A2, P.cDb, P.B1);
     } else {
@@ -506,24 +527,24 @@ __global__ __launch_bounds__(256) void k_bwd_blend(BwdArgs A) {
           const int px0 = tx * TS, py0 = ty * TS;
           const int pe = p0 + q / sp, ro = (sp == 4) ? (lane >> 4) * quarter_rows : (lane >> 5) * half_rows;
           if (masked && floor_tile)
-            bwd_rows_rec2<true, true, true>(pxa, pxb, pxc, p0, pe, ro, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+            bwd_rows_rec2<true, true, true>(pxa, pxb, pxc, p0, pe, ro, wv, px0, py0, jb.cx, jb.cy, k9, q2.y,
                                             q2.z, q2.w, b3, b2, kap, G);
           else if (masked)
-            bwd_rows_rec2<true, false, true>(pxa, pxb, pxc, p0, pe, ro, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+            bwd_rows_rec2<true, false, true>(pxa, pxb, pxc, p0, pe, ro, wv, px0, py0, jb.cx, jb.cy, k9, q2.y,
                                              q2.z, q2.w, b3, b2, kap, G);
           else
-            bwd_rows_rec2<false, false, true>(pxa, pxb, pxc, p0, pe, ro, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+            bwd_rows_rec2<false, false, true>(pxa, pxb, pxc, p0, pe, ro, wv, px0, py0, jb.cx, jb.cy, k9, q2.y,
                                               q2.z, q2.w, b3, b2, kap, G);
         } else if (rows) {
           const int px0 = tx * TS, py0 = ty * TS;
           if (masked && floor_tile)
-            bwd_rows_rec2<true, true, false>(pxa, pxb, pxc, p0, p1, 0, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+            bwd_rows_rec2<true, true, false>(pxa, pxb, pxc, p0, p1, 0, wv, px0, py0, jb.cx, jb.cy, k9, q2.y,
                                              q2.z, q2.w, b3, b2, kap, G);
           else if (masked)
-            bwd_rows_rec2<true, false, false>(pxa, pxb, pxc, p0, p1, 0, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+            bwd_rows_rec2<true, false, false>(pxa, pxb, pxc, p0, p1, 0, wv, px0, py0, jb.cx, jb.cy, k9, q2.y,
                                               q2.z, q2.w, b3, b2, kap, G);
           else
-            bwd_rows_rec2<false, false, false>(pxa, pxb, pxc, p0, p1, 0, wv, hc, px0, py0, jb.cx, jb.cy, k9, q2.y,
+            bwd_rows_rec2<false, false, false>(pxa, pxb, pxc, p0, p1, 0, wv, px0, py0, jb.cx, jb.cy, k9, q2.y,
                                                q2.z, q2.w, b3, b2, kap, G);
         } else {
           if (masked && floor_tile)

@@ -222,7 +222,7 @@ static fe_status plan(const fe_scene* scenes, int32_t n_scenes, const fe_view* v
   L->o_rank = p; p = align256(p + sizeof(int32_t) * (size_t)std::max<int64_t>(rec, 1));
   L->o_lmin = p; p = align256(p + sizeof(float) * 2 * tiles);   // thresholds, then floor seeds
   L->o_ubm = p; p = align256(p + sizeof(float) * tiles);
-  L->o_state = p; p = align256(p + sizeof(float) * 6 * pix);
+  L->o_state = p; p = align256(p + sizeof(float) * 7 * pix);   // M S Z E Zall D Mt
   if (o.mode == FE_MODE_CULLED) {
     L->o_mask = p; p = align256(p + sizeof(unsigned long long) * std::max<int64_t>(maskw, 1));
     L->o_mask2 = p; p = align256(p + sizeof(unsigned long long) * std::max<int64_t>(maskw, 1));

@@ -38,6 +38,25 @@ constexpr float kLog2e = 1.4426950408889634f;
 constexpr float kLn2 = 0.6931471805599453f;
 constexpr int kRecFloats = 16;
 constexpr int kRec2Floats = 16;
+// A blend record (kX = a0.xyz, kY1 = (a0.w, a1.x, a1.y), kY2 = (a1.z, a1.w, a2.x), the
+// last of each the constant term) with the constant terms moved to the point (ox, oy) of the
+// camera plane: the forms of p = (ox + x', oy + y', 1) become k.(x', y', 1).  Pixel-grid views
+// evaluate their forms this way in every pass (blend, backward, feature kernels), relative to
+// the centre of the 32x32-pixel block holding the pixel, with these operations: the passes
+// agree bit for bit, and so do the tile sizes (8, 16 and 32-pixel tiles lie inside one block).
+constexpr int kFormBlock = 32;
+constexpr float kFormHalf = 15.5f;   // (kFormBlock - 1) / 2
+// first pixel of the block holding pixel column (or row) p >= 0
+__device__ __forceinline__ int form_block(int p) { return p / kFormBlock * kFormBlock; }
+// the pixel's coordinate relative to its block's centre (exact)
+__device__ __forceinline__ float form_coord(int p) { return (float)(p - form_block(p)) - kFormHalf; }
+// the block centre's camera-plane coordinate (principal point c)
+__device__ __forceinline__ float form_origin(int p, float c) { return ((float)form_block(p) + kFormHalf) - c; }
+__device__ __forceinline__ void localize_rec2(float4& a0, float4& a1, float4& a2, float ox, float oy) {
+  a0.z = fmaf(a0.x, ox, fmaf(a0.y, oy, a0.z));
+  a1.y = fmaf(a0.w, ox, fmaf(a1.x, oy, a1.y));
+  a2.x = fmaf(a1.z, ox, fmaf(a1.w, oy, a2.x));
+}
 // Smallest |q|^2 the blend record form accepts (smaller: the view uses the direct form).
 constexpr double kMinQQ = 1e-20;
 

@@ -119,15 +119,32 @@ __global__ __launch_bounds__(TS * TS) void k_feat_fwd(FeatArgs A) {
   const float kap = r2 ? nb2l / b3 : 0.0f;
   const float4* rec4 = reinterpret_cast<const float4*>(r2 ? A.rec2 : A.rec) + jb.rec_base * 4;
   const int per = r2 ? 3 : 4;
+  // pixel-grid tiles with blend records: the blend's forms, relative to the pixel's 32x32
+  // block (fe_common.h, localize_rec2)
+  const bool loc = r2 && !jb.ray_xy;
+  const float ox = form_origin(tx * TS, jb.cx), oy = form_origin(ty * TS, jb.cy);
+  if (loc) {
+    xf = form_coord(tx * TS + (int)threadIdx.x % TS);
+    yf = form_coord(ty * TS + (int)threadIdx.x / TS);
+  }
 
   float M = -3.402823466e38f, Mt = -3.402823466e38f, S = 0.0f;
   float F[4] = {0.f, 0.f, 0.f, 0.f}, Fa[4] = {0.f, 0.f, 0.f, 0.f};
   bool nanf = false;
   for (int64_t base = 0; base < ncnt; base += kFeatBatch) {
     const int nb = (int)min<int64_t>(kFeatBatch, ncnt - base);
-    for (int k = threadIdx.x; k < nb * per; k += NT) {
-      const int e = k / per, part = k - e * per;
-      lrec[e * 4 + part] = rec4[list_record(A, L, base + e) * 4 + part];
+    if (loc) {
+      for (int e = threadIdx.x; e < nb; e += NT) {
+        const int64_t rid = list_record(A, L, base + e);
+        float4 a0 = rec4[rid * 4], a1 = rec4[rid * 4 + 1], a2 = rec4[rid * 4 + 2];
+        localize_rec2(a0, a1, a2, ox, oy);
+        lrec[e * 4] = a0; lrec[e * 4 + 1] = a1; lrec[e * 4 + 2] = a2;
+      }
+    } else {
+      for (int k = threadIdx.x; k < nb * per; k += NT) {
+        const int e = k / per, part = k - e * per;
+        lrec[e * 4 + part] = rec4[list_record(A, L, base + e) * 4 + part];
+      }
     }
     for (int e = threadIdx.x; e < nb; e += NT) lf[e] = feat_chunk(jb, gauss_of(jb, list_record(A, L, base + e)), c0);
     __syncthreads();
@@ -136,7 +153,7 @@ __global__ __launch_bounds__(TS * TS) void k_feat_fwd(FeatArgs A) {
       float t, sv;
       if (r2) {
         // the forward blend's operations (fe_forward.hip, blend_batch_rec2)
-        const float X = fmaf(r0.x, xf, fmaf(r0.y, yf, r0.z));
+        const float X = fmaf(r0.x, xf, fmaf(r0.y, yf, r0.z));   // (Y1, Y2 likewise)
         const float Y1 = fmaf(r0.w, xf, fmaf(r1.x, yf, r1.y));
         const float Y2 = fmaf(r1.z, xf, fmaf(r1.w, yf, r2v.x));
         const float W = fmaf(Y1, Y1, Y2 * Y2);
@@ -207,6 +224,7 @@ __global__ __launch_bounds__(256) void k_feat_bwd(FeatArgs A) {
   __shared__ float4 pxa[NPIX];   // x', y', M, 1/(S + 1e-20 n)
   __shared__ float4 pxg[NPIX];   // gF (this chunk)
   __shared__ float2 pxb[NPIX];   // gF.F (this chunk), the maximum-logit term
+  __shared__ float2 pxl[NPIX];   // x', y' relative to the pixel's 32x32 block (the blend's forms)
   __shared__ float4 red[4];
   const int64_t gt = blockIdx.x;
   const int ji = find_job(A.n_jobs, gt, [&](int k) { return A.jobs[k].tile_base; });
@@ -266,6 +284,7 @@ __global__ __launch_bounds__(256) void k_feat_bwd(FeatArgs A) {
     pxa[i] = make_float4(xf, yf, M, iS);
     pxg[i] = make_float4(gv[0], gv[1], gv[2], gv[3]);
     pxb[i] = make_float2(GF, cX);
+    pxl[i] = make_float2(form_coord(tx * TS + i % wv), form_coord(ty * TS + i / wv));
   }
   __syncthreads();
 
@@ -282,8 +301,13 @@ __global__ __launch_bounds__(256) void k_feat_bwd(FeatArgs A) {
     if (kk >= total) break;
     const int64_t rid = list_record(A, L, kk);
     const int64_t sg = gauss_of(jb, rid);
-    const float4 r0 = rec4[rid * 4], r1 = rec4[rid * 4 + 1], r2v = rec4[rid * 4 + 2];
+    float4 r0 = rec4[rid * 4], r1 = rec4[rid * 4 + 1], r2v = rec4[rid * 4 + 2];
     const float4 r3 = r2 ? make_float4(0.f, 0.f, 0.f, 0.f) : rec4[rid * 4 + 3];
+    // pixel-grid tiles with blend records: the blend's forms (relative to the pixel's 32x32
+    // block), so that s is the forward's weight bit for bit (the sums below stay in the
+    // pixels' own coordinates)
+    const bool loc = r2 && !jb.ray_xy;
+    if (loc) localize_rec2(r0, r1, r2v, form_origin(tx * TS, jb.cx), form_origin(ty * TS, jb.cy));
     const float4 fj = feat_chunk(jb, sg, c0);
     float G[13];
 #pragma unroll
@@ -298,9 +322,11 @@ __global__ __launch_bounds__(256) void k_feat_bwd(FeatArgs A) {
       float X = 0.f, Y1 = 0.f, Y2 = 0.f, rr = 0.f, Wr = 0.f, iv = 0.f;
       float u[3] = {0.f, 0.f, 0.f}, v[3] = {0.f, 0.f, 0.f};
       if (r2) {
-        X = fmaf(r0.x, xf, fmaf(r0.y, yf, r0.z));
-        Y1 = fmaf(r0.w, xf, fmaf(r1.x, yf, r1.y));
-        Y2 = fmaf(r1.z, xf, fmaf(r1.w, yf, r2v.x));
+        const float2 pl = pxl[p];
+        const float xe = loc ? pl.x : xf, ye = loc ? pl.y : yf;
+        X = fmaf(r0.x, xe, fmaf(r0.y, ye, r0.z));
+        Y1 = fmaf(r0.w, xe, fmaf(r1.x, ye, r1.y));
+        Y2 = fmaf(r1.z, xe, fmaf(r1.w, ye, r2v.x));
         const float W = fmaf(Y1, Y1, Y2 * Y2);
         rr = __builtin_amdgcn_rcpf(fmaf(X, X, W));
         t = X * rr;

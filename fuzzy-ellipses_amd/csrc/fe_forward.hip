@@ -1129,8 +1129,9 @@ struct BlendArgs {
   const unsigned long long* off1;
   const unsigned long long* off2;
   const uint32_t* list;
-  float* st;            // per-pixel state, 6 planes of total_pix: M S Z E Zall D (Zall and the
-                        // NaN flag in the sixth plane feed the second pass; the backward reads M S Z E D)
+  float* st;            // per-pixel state, 7 planes of total_pix: M S Z E Zall D Mt.  Zall, the
+                        // NaN flag (direct form, in the D plane) and the exact maximum Mt (blend
+                        // records) feed the second pass; the backward reads M S Z E D Mt
   int64_t total_pix;
   float* tile_lmin;
   float* tile_seed;     // phase 1 writes the floor seed of each tile (phase-2 membership input)
@@ -1220,6 +1221,17 @@ void k_blend(BlendArgs A) {
   const Job& jb = A.jobs[ji];
   const int tile = (int)(gt - jb.tile_base);
   const int tx = tile % jb.tiles_x, ty = tile / jb.tiles_x;
+  const int32_t jflags = A.job_flags[ji];
+  const bool safe = (jflags & kFlagDegenerate) != 0;   // the view has degenerate records
+  const bool r2 = use_rec2(jb, jflags);
+  // Pixel-grid tiles with blend records evaluate the forms in coordinates relative to the
+  // centre (ox, oy) of their 32x32-pixel block (form_coord), with each record's constant
+  // terms moved there when it is staged (localize_rec2): the backward evaluates the same
+  // forms with the same operations, so every t*, d2 and weight it recomputes is this pass's
+  // value bit for bit.  Ray lists and the direct form use the rays' or pixels' own
+  // coordinates.
+  const bool loc = r2 && !jb.ray_xy;
+  const float ox = form_origin(tx * TS, jb.cx), oy = form_origin(ty * TS, jb.cy);
   // pixel q of this thread: tile pixel index PPT*ti + q, so a thread's pixels are neighbours
   // in one row (same y: the blend records' y-parts are shared between them)
   bool inimg[PPT];
@@ -1236,6 +1248,9 @@ void k_blend(BlendArgs A) {
     if (jb.ray_xy) {
       xf[q] = inimg[q] ? jb.ray_xy[2 * lp] : 0.0f;
       yf[q] = inimg[q] ? jb.ray_xy[2 * lp + 1] : 0.0f;
+    } else if (loc) {
+      xf[q] = form_coord(px);
+      yf[q] = form_coord(py);
     } else {
       xf[q] = (float)px - jb.cx;
       yf[q] = (float)py - jb.cy;
@@ -1244,9 +1259,6 @@ void k_blend(BlendArgs A) {
   const float b3 = jb.beta3;
   const float nb2l = -jb.beta2 * kLog2e;
   const float kd = -0.5f * kLog2e;
-  const int32_t jflags = A.job_flags[ji];
-  const bool safe = (jflags & kFlagDegenerate) != 0;   // the view has degenerate records
-  const bool r2 = use_rec2(jb, jflags);
   const float kap = r2 ? nb2l / b3 : 0.0f;
   const bool masked = (PHASE == 0) || !A.tile_np || A.tile_np[2 * gt + (PHASE == 2 ? 1 : 0)] != 0;
 
@@ -1267,13 +1279,19 @@ void k_blend(BlendArgs A) {
       M[q] = -3.402823466e38f; S[q] = 0.f; Z[q] = 0.f; E[q] = 0.f; Zall[q] = 0.f;
       nanf[q] = false;
     }
-    Mt[q] = (r2 && PHASE == 2) ? M[q] / b3 : -3.402823466e38f;
+    Mt[q] = -3.402823466e38f;
   }
 
   int64_t start, end;
   if (PHASE == 0) { start = 0; end = jb.n; }
   else if (PHASE == 1) { start = (int64_t)A.off1[gt]; end = (int64_t)A.off1[gt + 1]; }
   else { start = (int64_t)A.off2[gt]; end = (int64_t)A.off2[gt + 1]; }
+  // a second pass with pairs continues from the first pass's exact maximum (views with blend
+  // records; without pairs it changes neither the maximum nor that plane)
+  if (PHASE == 2 && r2 && end > start) {
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) Mt[q] = A.st[6 * A.total_pix + jb.pix_base + lpx[q]];
+  }
   const float4* rec4 = reinterpret_cast<const float4*>(A.rec) + jb.rec_base * 4;
   const float4* rec24 = reinterpret_cast<const float4*>(A.rec2) + jb.rec_base * 4;
   // (a tile lists each record of its view at most once: its list length fits 32 bits)
@@ -1294,8 +1312,50 @@ void k_blend(BlendArgs A) {
          : (part == 3) ? make_float4(-INFINITY, -INFINITY, 0.f, 0.f)
                        : make_float4(0.f, 0.f, 0.f, 0.f);
   };
+  // a blend record staged with its constant terms at the block's centre (a null record stays
+  // null: X = 0, Y1 = 1, wL = -inf)
+  auto stage_local = [&](int k, bool real, int64_t gid) {
+    float4 a0, a1, a2;
+    if (real) {
+      a0 = rec24[gid * 4];
+      a1 = rec24[gid * 4 + 1];
+      a2 = rec24[gid * 4 + 2];
+    } else {
+      a0 = make_float4(0.f, 0.f, 0.f, 0.f);
+      a1 = make_float4(0.f, 1.f, 0.f, 0.f);
+      a2 = make_float4(0.f, -INFINITY, 0.f, 0.f);
+    }
+    localize_rec2(a0, a1, a2, ox, oy);
+    lds[k * 3] = a0;
+    lds[k * 3 + 1] = a1;
+    lds[k * 3 + 2] = a2;
+  };
   auto stage = [&](auto pc, int v, int o, int len) {
     constexpr int P = decltype(pc)::value;
+    if constexpr (P == 3) {
+      if (loc) {   // one record per thread (its three parts together)
+        if constexpr (KP == 1) {
+          const int cnt = min(R, len - o), nbr = (cnt + SUB - 1) / SUB * SUB;
+          const int li0 = v * h + o;
+          for (int k = threadIdx.x; k < nbr; k += NB) {
+            const bool real = k < cnt;
+            const int64_t gid = !real ? 0 : (PHASE == 0) ? (int64_t)(li0 + k) : (int64_t)A.list[start + li0 + k];
+            stage_local(k, real, gid);
+          }
+        } else {
+          for (int k = threadIdx.x; k < KP * R; k += NB) {
+            const int g = k / R, kk = k - g * R;
+            const int cg = min(R, range_len(g) - o);
+            if (kk >= (cg + SUB - 1) / SUB * SUB) continue;
+            const bool real = kk < cg;
+            const int li = g * h + o + kk;
+            const int64_t gid = !real ? 0 : (PHASE == 0) ? (int64_t)li : (int64_t)A.list[start + li];
+            stage_local(k, real, gid);
+          }
+        }
+        return;
+      }
+    }
     if constexpr (KP == 1) {
       // one group: records [o, o + cnt) of range v, padded to the last SUB group
       const int cnt = min(R, len - o), nbr = (cnt + SUB - 1) / SUB * SUB;
@@ -1373,7 +1433,7 @@ void k_blend(BlendArgs A) {
         m[2 * NT * PPT + pi] = Z[q];
         m[3 * NT * PPT + pi] = E[q];
         m[4 * NT * PPT + pi] = Zall[q];
-        m[5 * NT * PPT + pi] = nanf[q] ? 1.0f : 0.0f;
+        m[5 * NT * PPT + pi] = r2 ? Mt[q] : (nanf[q] ? 1.0f : 0.0f);   // (the NaN flag: direct form only)
         M[q] = -3.402823466e38f; S[q] = 0.f; Z[q] = 0.f; E[q] = 0.f; Zall[q] = 0.f;
         nanf[q] = false;
         Mt[q] = -3.402823466e38f;
@@ -1387,7 +1447,7 @@ void k_blend(BlendArgs A) {
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
       const int pi = q * NT + ti;
-      float Mm = M[q];
+      float Mm = M[q], Mtm = Mt[q];
 #pragma unroll
       for (int k = 1; k < KS; ++k) Mm = fmaxf(Mm, mrg[(k - 1) * 6][pi]);
       // equal maxima (infinite ones included) keep their sums as they are
@@ -1403,9 +1463,11 @@ void k_blend(BlendArgs A) {
         Z[q] = fmaf(m[2 * NT * PPT + pi], sk, Z[q]);
         E[q] += m[3 * NT * PPT + pi];
         Zall[q] += m[4 * NT * PPT + pi];
-        nanf[q] = nanf[q] || m[5 * NT * PPT + pi] != 0.0f;
+        if (r2) Mtm = fmaxf(Mtm, m[5 * NT * PPT + pi]);   // M = b3 Mt rounded is monotone in Mt
+        else nanf[q] = nanf[q] || m[5 * NT * PPT + pi] != 0.0f;
       }
       M[q] = Mm;
+      Mt[q] = Mtm;
     }
   }
 
@@ -1428,6 +1490,7 @@ void k_blend(BlendArgs A) {
         A.st[3 * A.total_pix + jb.pix_base + lpx[q]] = E[q];
         A.st[4 * A.total_pix + jb.pix_base + lpx[q]] = Zall[q];
         if (safe) A.st[5 * A.total_pix + jb.pix_base + lpx[q]] = nanf[q] ? 1.0f : 0.0f;
+        if (r2) A.st[6 * A.total_pix + jb.pix_base + lpx[q]] = Mt[q];
         const bool pos = S[q] > 0.0f && !nanf[q];
         const float l2s = pos ? __log2f(S[q]) : 0.0f;
         lm = fminf(lm, pos ? M[q] + l2s : -INFINITY);
@@ -1465,12 +1528,14 @@ void k_blend(BlendArgs A) {
       jb.depth[lpx[q]] = D;
       jb.alpha[lpx[q]] = alpha;
       if (A.save) {
-        // what the backward reads: M, S, Z, E (unchanged by a second pass with no pairs) and D
+        // what the backward reads: M, S, Z, E and, with blend records, the exact maximum Mt
+        // (all unchanged by a second pass with no pairs), and D
         if (PHASE == 0 || end > start) {
           A.st[jb.pix_base + lpx[q]] = M[q];
           A.st[A.total_pix + jb.pix_base + lpx[q]] = S[q];
           A.st[2 * A.total_pix + jb.pix_base + lpx[q]] = Z[q];
           A.st[3 * A.total_pix + jb.pix_base + lpx[q]] = E[q];
+          if (r2) A.st[6 * A.total_pix + jb.pix_base + lpx[q]] = Mt[q];
         }
         A.st[5 * A.total_pix + jb.pix_base + lpx[q]] = D;
       }

@@ -148,6 +148,55 @@ def render_camera(means, prec_full, weights_log, R, t, camera_rays, beta_2, beta
                             weights_log, beta_2, beta_3)
 
 
+def depth_condition(rays, starts, means, prec_full, weights_log, beta_2, beta_3, t_error=True):
+    """First-order sensitivity of each ray's depth to float32 rounding of the per-pair math,
+    kappa (P,): a float32 evaluation of render_func_rays, by any arrangement of its
+    operations, moves depth by about 2^-24 * (a small constant) * kappa, however exact the
+    evaluation is otherwise.  Test infrastructure: it sets how far float32 can be expected to
+    follow the float64 result where the 1e-4 relative depth budget is out of float32's reach
+    (logits of thousands: pixels far, in whitened units, from every Gaussian; t* near 0 on
+    rays almost perpendicular to a Gaussian's direction).
+
+    With s_j the unmasked softmax weights and f the 1e-20 floor (fm_render.py:93-99), depth
+    D = sum (s_j + f) z_j / sum (s_j + f), z = nan_to_num(t*).  A logit error dl_j moves D by
+    s_j (z_j - D) dl_j, an error in the maximum logit (util_render.py:5-9) by
+    -sum_j s_j (z_j - D) dl_max, and an error dz_j by (s_j + f) dz_j.  The error scales are
+      dz_j  = |q_j| / |A_j r|           (t* = (q.Ar)/|Ar|^2, q = A(mu - o): the cancellation
+                                         in q.Ar when the ray is nearly perpendicular to q)
+      dl_j  = b3 (|w_j| + d0_j + |q_j| sqrt(d0_j)) + b2 dz_j
+                                        (d2 = w - d0/2, d0 = |A(t* r - p)|^2: rounding of
+                                         t* r - p at the scale of |q|)
+    and kappa = sum s_j |z_j - D| dl_j + sum (s_j + f) dz_j + |sum s_j (z_j - D)| dl_max,
+    all in float64 from baseline/fm_render.py:22-99's quantities.  ``t_error=False`` leaves
+    the t* terms out (dz = 0); the tests use it to show that a kappa missing them fails.
+    """
+    rays = np.asarray(rays, dtype=np.float64)
+    starts = np.broadcast_to(np.asarray(starts, dtype=np.float64), rays.shape)
+    mu = np.asarray(means, dtype=np.float64)
+    w = np.asarray(weights_log, dtype=np.float64)
+    zs, d2 = pair_terms(rays, starts, mu, prec_full, w)
+    prc = np.transpose(np.triu(np.asarray(prec_full, dtype=np.float64)), (0, 2, 1))
+    q = np.einsum("nij,pnj->pni", prc, mu[None, :, :] - starts[:, None, :])
+    qn = np.sqrt(np.sum(q * q, axis=-1))                                      # (P,N)
+    pr = np.einsum("nij,pj->pni", prc, rays)
+    with np.errstate(all="ignore"):
+        dz = qn / np.sqrt(np.sum(pr * pr, axis=-1)) if t_error else np.zeros_like(qn)
+        d0 = np.maximum(2.0 * (w[None, :] - d2), 0.0)
+        logit = -zs * beta_2 + beta_3 * d2
+        lmax = np.max(np.where(np.isnan(logit), -np.inf, logit), axis=1, keepdims=True)
+        se = (zs > 0) * np.nan_to_num(np.exp(logit - lmax))
+        den = np.sum(se, axis=1, keepdims=True) + 1e-20 * zs.shape[1]
+        s = se / den
+        f = 1e-20 / den
+        z = np.nan_to_num(zs)
+        D = np.sum((s + f) * z, axis=1, keepdims=True)
+        dl = beta_3 * (np.abs(w)[None, :] + d0 + qn * np.sqrt(d0)) + beta_2 * dz
+        am = np.argmax(np.where(np.isnan(logit), -np.inf, logit), axis=1)
+        dl_max = dl[np.arange(dl.shape[0]), am]
+        return (np.sum(s * np.abs(z - D) * dl, axis=1) + np.sum((s + f) * dz, axis=1)
+                + np.abs(np.sum(s * (z - D), axis=1)) * dl_max)
+
+
 def idx_quattrans_rays(pixel_posei, invF, poses):
     """World rays and starts of baseline/fm_render.py:151-166: per ray, the camera-frame ray
     (x invF, y invF, z) @ R(pose), starting at the pose's translation."""

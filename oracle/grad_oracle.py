@@ -71,25 +71,29 @@ def feature_grads(scene, features, cams, d_depth, d_alpha, d_features, chunk=204
     return means.grad.numpy(), np.triu(prec.grad.numpy()), w.grad.numpy(), f.grad.numpy()
 
 
-def grads(scene, cams, d_depth, d_alpha, chunk=2048, pixels=None):
+def grads(scene, cams, d_depth, d_alpha, chunk=2048, pixels=None, dtype=torch.float64):
     """Gradients of sum_v <d_depth_v, depth_v> + <d_alpha_v, alpha_v> (float64).
 
     With ``pixels`` (one index array per view) only those pixels are rendered, and
     d_depth[v] / d_alpha[v] hold their cotangents (the others being zero): the render is
     separable per pixel, so this is the gradient of the full sum with zeros elsewhere.
-    Returns (d_means (N,3), d_prec (N,3,3) -- upper triangle, d_weights_log (N,))."""
-    means = torch.tensor(scene.means, dtype=torch.float64, requires_grad=True)
-    prec = torch.tensor(scene.prec, dtype=torch.float64, requires_grad=True)
-    w = torch.tensor(scene.weights_log, dtype=torch.float64, requires_grad=True)
+    ``dtype=torch.float32`` evaluates the same autograd in float32, the precision the
+    reference's JAX path differentiates in by default: its distance from the float64 result
+    measures how well float32 can resolve these gradients at all (inputs, rays and the
+    cotangents are rounded to float32 first).
+    Returns (d_means (N,3), d_prec (N,3,3) -- upper triangle, d_weights_log (N,)) in float64."""
+    def leaf(x):
+        return torch.tensor(np.asarray(x, dtype=np.float64), dtype=dtype, requires_grad=True)
+    means, prec, w = leaf(scene.means), leaf(scene.prec), leaf(scene.weights_log)
     for v, cam in enumerate(cams):
         rays_all = pinhole_rays(cam, None if pixels is None else pixels[v])
         gd = np.asarray(d_depth[v], dtype=np.float64).reshape(-1)
         ga = np.asarray(d_alpha[v], dtype=np.float64).reshape(-1)
         for s in range(0, rays_all.shape[0], chunk):
             e = min(rays_all.shape[0], s + chunk)
-            rays = torch.tensor(rays_all[s:e])
-            starts = torch.tensor(np.repeat(np.asarray(cam.t, dtype=np.float64)[None], e - s, 0))
+            rays = torch.tensor(rays_all[s:e], dtype=dtype)
+            starts = torch.tensor(np.repeat(np.asarray(cam.t, dtype=np.float64)[None], e - s, 0), dtype=dtype)
             d, a = render_rays_torch(rays, starts, means, prec, w, scene.beta2, scene.beta3)
-            loss = (d * torch.tensor(gd[s:e])).sum() + (a * torch.tensor(ga[s:e])).sum()
+            loss = (d * torch.tensor(gd[s:e], dtype=dtype)).sum() + (a * torch.tensor(ga[s:e], dtype=dtype)).sum()
             loss.backward()
-    return means.grad.numpy(), np.triu(prec.grad.numpy()), w.grad.numpy()
+    return (means.grad.double().numpy(), np.triu(prec.grad.double().numpy()), w.grad.double().numpy())
