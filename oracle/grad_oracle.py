@@ -97,3 +97,20 @@ def grads(scene, cams, d_depth, d_alpha, chunk=2048, pixels=None, dtype=torch.fl
             loss = (d * torch.tensor(gd[s:e], dtype=dtype)).sum() + (a * torch.tensor(ga[s:e], dtype=dtype)).sum()
             loss.backward()
     return (means.grad.double().numpy(), np.triu(prec.grad.double().numpy()), w.grad.double().numpy())
+
+
+def ray_grads(scene, ray_views, d_depth, d_alpha, dtype=torch.float64):
+    """grads() for views given as explicit rays: ray_views = [(rays (m,3) world directions,
+    origin (3,)), ...] with one cotangent pair per view (the ray-list views of the renderer;
+    their rays are (x, y, 1) @ R of the pose)."""
+    def leaf(x):
+        return torch.tensor(np.asarray(x, dtype=np.float64), dtype=dtype, requires_grad=True)
+    means, prec, w = leaf(scene.means), leaf(scene.prec), leaf(scene.weights_log)
+    for (rays, origin), gd, ga in zip(ray_views, d_depth, d_alpha):
+        r = torch.tensor(np.asarray(rays, dtype=np.float64), dtype=dtype)
+        st = torch.tensor(np.repeat(np.asarray(origin, dtype=np.float64)[None], r.shape[0], 0), dtype=dtype)
+        d, a = render_rays_torch(r, st, means, prec, w, scene.beta2, scene.beta3)
+        loss = (d * torch.tensor(np.asarray(gd, dtype=np.float64), dtype=dtype)).sum() + \
+            (a * torch.tensor(np.asarray(ga, dtype=np.float64), dtype=dtype)).sum()
+        loss.backward()
+    return (means.grad.double().numpy(), np.triu(prec.grad.double().numpy()), w.grad.double().numpy())

@@ -71,3 +71,23 @@ def build(cfg, sc):
     scene = sc.Scene(_f32(means), _f32(prec), _f32(w), cams, beta2=float(np.float32(cfg["beta2"])),
                      beta3=float(np.float32(cfg["beta3"])))
     return scene, cams
+
+
+def ray_lists(cfg, cams, max_rays=3000):
+    """One list of camera-plane rays (m, 2) float32 per camera: sub-pixel positions over the
+    image, or over three times its extent (rays far off the view), in Morton order or in
+    random order; from the configuration's seed."""
+    from fuzzy_ellipses_amd.rays import _morton_order
+    rng = np.random.default_rng([cfg["seed"], 7])
+    out = []
+    for cam in cams:
+        m = int(rng.integers(1, max_rays + 1))
+        wide = bool(rng.integers(0, 2))
+        lo_x, hi_x = (-1.5 * cam.W, 2.5 * cam.W) if wide else (-0.5, cam.W - 0.5)
+        lo_y, hi_y = (-1.5 * cam.H, 2.5 * cam.H) if wide else (-0.5, cam.H - 0.5)
+        x, y = rng.uniform(lo_x, hi_x, m), rng.uniform(lo_y, hi_y, m)
+        xy = np.stack([(x - cam.cx) / cam.fx, (y - cam.cy) / cam.fy], 1).astype(np.float32)
+        if rng.integers(0, 2):
+            xy = xy[_morton_order(xy[:, 0].astype(np.float64), xy[:, 1].astype(np.float64))]
+        out.append(np.ascontiguousarray(xy))
+    return out
