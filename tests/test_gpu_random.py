@@ -105,6 +105,36 @@ def test_forward_random_configs(fea, scenes_mod):
     assert not bad, bad[:5]
 
 
+def test_random_multi_scene_calls_equal_lone_views(fea, scenes_mod):
+    """Groups of four random configurations in one call (their scenes, pixel-grid views and ray
+    lists; the group's first configuration sets tile, batch depth and mode): every view equals
+    its own single-view call bit for bit."""
+    import torch
+    bad = []
+    for g in range(25):
+        cfgs = [randcfg.draw(1000 + 4 * g + k) for k in range(4)]
+        base = cfgs[0]
+        scenes, views = [], []
+        for i, cfg in enumerate(cfgs):
+            scene, cams = randcfg.build(cfg, scenes_mod)
+            scenes.append(fea.SceneTensors.from_scene(scene))
+            views += [(i, c, scene.beta2, scene.beta3) for c in cams]
+            views += [(i, fea.RayList(c.R, c.t, torch.tensor(xy, device="cuda").contiguous()), scene.beta2, scene.beta3)
+                      for c, xy in zip(cams, randcfg.ray_lists(cfg, cams, max_rays=800))]
+        r = fea.Renderer(tile=base["tile"], batch=base["batch"], mode=base["mode"])
+        outs = r.forward(scenes, views)
+        torch.cuda.synchronize()
+        for v, (i, cam, b2, b3) in enumerate(views):
+            (d1, a1), = r.forward([scenes[i]], [(0, cam, b2, b3)])
+            torch.cuda.synchronize()
+            d0, a0 = outs[v]
+            if not (torch.equal(torch.nan_to_num(d0, 7.0), torch.nan_to_num(d1, 7.0))
+                    and torch.equal(torch.nan_to_num(a0, 7.0), torch.nan_to_num(a1, 7.0))):
+                bad.append((g, v))
+    print(f"multi-scene calls: {25 - len({g for g, _ in bad})}/25 groups equal their lone views bit for bit")
+    assert not bad, bad[:5]
+
+
 def _grads_ok(got, ref64, ref32_fn, mass):
     """None if each gradient array meets the bounds against float64 autograd, else what failed."""
     ref32 = None
