@@ -494,7 +494,13 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
   __syncthreads();
   const float tau_s = (PHASE == 2) ? tau_grp : 0.0f;
   __shared__ unsigned long long wmask[NS][kWinWords];   // this window's membership words
-  __shared__ unsigned long long wlist[NS][kWinWords];   // phase 2: the phase-1 words of the window
+  // phase 2: the phase-1 words of the window (phase 1 has none: its block then fits 7 times
+  // per CU instead of 6)
+  __shared__ unsigned long long wlist[PHASE == 2 ? NS : 1][kWinWords];
+  auto listed_in_1 = [&](int k, int c, int bit) {
+    if constexpr (PHASE == 2) return ((wlist[k][c] >> bit) & 1ull) != 0ull;
+    else return false;
+  };
   __shared__ uint32_t queue[kWinWords * 64];            // group survivors (Gaussian ids)
   __shared__ uint32_t qn;
   __shared__ uint32_t tcnt[NS], tnp[NS];
@@ -517,7 +523,7 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
         const int64_t tile = (int64_t)(ty0 + k / nsx) * jb.tiles_x + (tx0 + k % nsx);
         lw = mask1[jb.mask_base + tile * jb.nw + w0 + c];
       }
-      wlist[k][c] = lw;
+      if constexpr (PHASE == 2) wlist[k][c] = lw;
     }
     if (threadIdx.x == 0) qn = 0;
     __syncthreads();
@@ -560,14 +566,14 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
 #pragma unroll
         for (int k = 0; k < NS; ++k) {
           if (k < nsub) {
-            const bool listed = (wlist[k][c] >> lane) & 1ull;
+            const bool listed = listed_in_1(k, c, lane);
             if (!listed && !plus_s && (single || !keep_s)) ubm[k] = fmaxf(ubm[k], ubm_s);
           }
         }
       }
       if (single) {
         // the group bound is the tile bound
-        const bool keep = (PHASE == 1) ? (forced || keep1) : (keep1 && !((wlist[0][c] >> lane) & 1ull));
+        const bool keep = (PHASE == 1) ? (forced || keep1) : (keep1 && !listed_in_1(0, c, lane));
         const unsigned long long word = __ballot(keep);
         const unsigned long long npw = __ballot(keep && !front_s);   // forced pairs: front_s false
         if (lane == 0) {
@@ -588,7 +594,7 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
         bool all_listed = true;
 #pragma unroll
         for (int k = 0; k < NS; ++k)
-          if (k < nsub) all_listed = all_listed && ((wlist[k][c] >> lane) & 1ull);
+          if (k < nsub) all_listed = all_listed && listed_in_1(k, c, lane);
         need = !all_listed;
       }
       const unsigned long long sw = __ballot(need);
@@ -624,7 +630,7 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
 #pragma unroll kSubUnroll
         for (int k = 0; k < NS; ++k) {
           if (k >= nsub) continue;
-          if (PHASE == 2 && ((wlist[k][c] >> bit) & 1ull)) continue;   // listed in phase 1
+          if (listed_in_1(k, c, bit)) continue;   // listed in phase 1
           const PairBound pb = pair_bound(r, tg[k]);
           bool keep;
           if (PHASE == 1) {
