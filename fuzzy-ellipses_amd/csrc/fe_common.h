@@ -180,8 +180,8 @@ constexpr int kCountSlots = 64;
 // phase-2 lower bound for tiles that phase 1 left without a threshold.
 constexpr int kMaskChunkBlocks = 1024;
 // The blend splits the tile lists of views of fewer 16-pixel tiles than this (8-pixel tiles:
-// every view; 32-pixel tiles: none).
-constexpr int kSplitMinTiles = 1024;
+// every view; 32-pixel tiles: none): up to 1280x720 and 960x1024.
+constexpr int kSplitMinTiles = 4096;
 __host__ __device__ inline bool view_splits_lists(int ts, int64_t n_tiles) {
   return ts == 8 || (ts == 16 && n_tiles < kSplitMinTiles);
 }

@@ -1155,11 +1155,12 @@ struct BlendArgs {
 // contiguous range [v h, min(L, (v + 1) h)) of it (h = L/KS rounded up to a multiple of SUB)
 // from its own empty state, and the KS states are merged at the end (max, then S and Z
 // rescaled to it; E and Zall added, NaN flags or-ed).  Views of kSplitMinTiles 16-pixel tiles
-// or more are not split (view_splits_lists): their grids fill the GPU by themselves in a batch, where the split's
-// instructions cost (the 16-view 480x640 shell: phase-1 blend 0.87-0.89 ms unsplit against
-// 0.92-0.94 split), while it halves a small view's latency (one 128x128 view, N=4096:
-// 0.74 -> 0.35 ms).  Whether a tile is split depends on its view alone, so a view's result does
-// not depend on the batch around it.  How a launch runs (MODE): 0 when no view of the batch is
+// or more are not split (view_splits_lists): their grids fill the GPU by themselves.  The split
+// halves a view's latency where its longest tiles set it (one 128x128 view, N=4096: 0.74 ->
+// 0.35 ms; one 480x640 view 0.63 -> 0.44 ms per forward call) and costs 1-2% in a batch that
+// fills the GPU many times (the 16-view 480x640 shell's phase-1 blend: the second range
+// restarts the online maximum, and the merge).  Whether a tile is split depends on its view
+// alone, so a view's result does not depend on the batch around it.  How a launch runs (MODE): 0 when no view of the batch is
 // split (the plain kernel), else 2, side by side on KS groups of NT threads, each round staging
 // R = GB/KS records of every range (a tile runs on KS times the waves, for grids whose time is
 // set by their longest tiles' latency), or 1, one after another on one group in rounds of GB
@@ -1408,7 +1409,10 @@ void k_blend(BlendArgs A) {
       else stage(std::integral_constant<int, 4>{}, v, o, len);
       __syncthreads();
       const int cnt = max(0, min(R, len - o));
-      const int sq = (cnt + SUB - 1) / SUB * SUB;
+      // records this round, padded (block-uniform; made explicit in the split modes, whose
+      // record loops otherwise count in vector registers: one more VALU per record)
+      int sq = (cnt + SUB - 1) / SUB * SUB;
+      if (MODE != 0) sq = __builtin_amdgcn_readfirstlane(sq);
       const float4* sl = lds + split * R * parts;   // this group's slot
       if (r2) {
         if (PPT > 1 && !jb.ray_xy) {

@@ -1,5 +1,5 @@
-"""Split tile lists: the blend cuts each tile list of a small view (fewer than 1024 16-pixel
-tiles; every view on 8-pixel tiles) into fixed contiguous ranges (two on 16-pixel tiles, four on
+"""Split tile lists: the blend cuts each tile list of a view of fewer than 4096 16-pixel tiles
+(every view on 8-pixel tiles) into fixed contiguous ranges (two on 16-pixel tiles, four on
 8-pixel tiles), blends them from separate states and merges the states.  On 16-pixel tiles the
 launcher runs the ranges side by side for grids up to 8 times what the GPU holds at once (28
 waves per CU; 57344 waves on 256 CUs) and one after another beyond that (8-pixel tiles always
@@ -51,14 +51,15 @@ def test_large_batch_equals_lone_view(fea, scenes_mod, mode, n, tile):
 
 
 def test_mixed_split_and_unsplit_views(fea, scenes_mod):
-    """A batch of split small views (128x128) and unsplit large ones (480x640, 1200 tiles): every
-    view equals its lone render bit for bit (the lone 480x640 view runs the plain kernel, the
-    batch the split one)."""
+    """A batch of split views (128x128 and 480x640) and unsplit large ones (1024x1024, 4096
+    tiles): every view equals its lone render bit for bit (the lone 1024x1024 view runs the plain
+    kernel, the batch the split one)."""
     import torch
     rng = np.random.default_rng(23)
     means, prec, w = scenes_mod.random_gaussians(rng, 2048)
     cams = [scenes_mod.make_camera(3.0 * scenes_mod.random_dir(rng), 128, 128, 120.0) for _ in range(3)]
-    cams += [scenes_mod.make_camera(3.0 * scenes_mod.random_dir(rng), 480, 640, 600.0) for _ in range(2)]
+    cams += [scenes_mod.make_camera(3.0 * scenes_mod.random_dir(rng), 480, 640, 600.0)]
+    cams += [scenes_mod.make_camera(3.0 * scenes_mod.random_dir(rng), 1024, 1024, 960.0) for _ in range(2)]
     scene = scenes_mod.Scene(means, prec, w, cams)
     st = fea.SceneTensors.from_scene(scene)
     views = [(0, c, scene.beta2, scene.beta3) for c in cams]
