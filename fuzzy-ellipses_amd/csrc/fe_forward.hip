@@ -1409,10 +1409,11 @@ void k_blend(BlendArgs A) {
       else stage(std::integral_constant<int, 4>{}, v, o, len);
       __syncthreads();
       const int cnt = max(0, min(R, len - o));
-      // records this round, padded (block-uniform; made explicit in the split modes, whose
-      // record loops otherwise count in vector registers: one more VALU per record)
+      // records this round, padded (block-uniform; made explicit in the split modes of 16- and
+      // 32-pixel tiles, whose record loops otherwise count in vector registers: one more VALU
+      // per record; 8-pixel tiles measured slower with it)
       int sq = (cnt + SUB - 1) / SUB * SUB;
-      if (MODE != 0) sq = __builtin_amdgcn_readfirstlane(sq);
+      if (MODE != 0 && PPT > 1) sq = __builtin_amdgcn_readfirstlane(sq);
       const float4* sl = lds + split * R * parts;   // this group's slot
       if (r2) {
         if (PPT > 1 && !jb.ray_xy) {
