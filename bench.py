@@ -30,8 +30,9 @@ TRANS_PEAK_TOPS = 9.8             # 256 CUs x 4 SIMDs x 16 lanes / 4 (quarter ra
 HBM_PEAK_GBS = 8000.0             # MI355X published HBM3E bandwidth
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
                             "traffic_shell_n4096_16x480x640.json")
-# kernel names in the profile (the 16-view workload's 480x640 views do not split their lists: mode 0)
-PROFILE_NAMES = {"bwd_blend": "fe::k_bwd_blend<16>", "blend1": "fe::k_blend<16, 64, 1, 0>",
+# kernel names in the profile (the 16-view workload's 480x640 views split their phase-1 lists,
+# one range after the other on a grid this size: mode 1; the second phase never splits)
+PROFILE_NAMES = {"bwd_blend": "fe::k_bwd_blend<16>", "blend1": "fe::k_blend<16, 64, 1, 1>",
                  "blend2": "fe::k_blend<16, 64, 2, 0>"}
 
 
