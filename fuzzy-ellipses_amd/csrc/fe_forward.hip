@@ -347,13 +347,56 @@ __device__ void finish_tile(const Job& jb, int64_t tile, uint32_t total, uint32_
   }
 }
 
-// Per lane, over the Gaussians of words [w0, w1) (lane = Gaussian, waves interleaved), the
-// largest lower bound b3 (w - sd_hi^2/2) - b2 t_ub of the logit at every pixel of tile tg of a
-// Gaussian entirely in front of the tile's rays; -inf if none.
-__device__ __forceinline__ float front_logit_lb(const Job& jb, const float* __restrict__ rec, const TileGeo& tg,
-                                                int w0, int w1, int wave, int lane, float b3l, float b2l) {
-  float lb = -INFINITY;
-  for (int c = w0 + wave; c < w1; c += 4) {
+// Phase 2 of a chunked membership (chunks > 1): for every tile of a group whose phase-1 lmin is
+// -inf, the largest lower bound b3 (w - sd_hi^2/2) - b2 t_ub of the logit at every pixel of the
+// tile over the Gaussians of this block's chunk that lie entirely in front of the tile's rays
+// (-inf if none), to lbpart[(group * chunks + chunk) * kMaskMaxSub + tile of the group]
+// (k_tile_mask takes the max over the chunks).  One pass over the chunk serves all of the
+// group's such tiles: each record is loaded once and bounded against each of them.
+template <int SFMAX>
+__global__ __launch_bounds__(256) void k_mask_lb(const Job* __restrict__ jobs, int n_jobs, int ts,
+                                                 const float* __restrict__ rec, const float* __restrict__ tile_lmin,
+                                                 const int32_t* __restrict__ job_flags, int chunks,
+                                                 float* __restrict__ lbpart) {
+  constexpr int NS = SFMAX * SFMAX;
+  static_assert(NS <= kMaskMaxSub, "partials hold kMaskMaxSub tiles per group");
+  const int64_t gs = blockIdx.x;
+  const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
+  const Job& jb = jobs[ji];
+  if ((job_flags[ji] & 1) != 0 || !bounds_apply(jb)) return;   // dense job: no thresholds
+  const int sf = jb.sf;
+  const int st = (int)(gs - jb.stile_base);
+  const int sx = st % jb.stiles_x, sy = st / jb.stiles_x;
+  const int tx0 = sx * sf, ty0 = sy * sf;
+  const int nsx = min(sf, jb.tiles_x - tx0), nsy = min(sf, jb.tiles_y - ty0);
+  const int nsub = nsx * nsy;
+  __shared__ TileGeo tg[NS];
+  __shared__ int unc[NS];
+  __shared__ int any_unc;
+  __shared__ float sh_lb[4][NS];
+  if (threadIdx.x == 0) any_unc = 0;
+  __syncthreads();
+  if (threadIdx.x < NS) {
+    const int k = threadIdx.x;
+    int u = 0;
+    if (k < nsub) {
+      const int txk = tx0 + k % nsx, tyk = ty0 + k / nsx;
+      u = tile_lmin[jb.tile_base + (int64_t)tyk * jb.tiles_x + txk] == -INFINITY;
+      tg[k] = tile_geo_xy(jb, txk, tyk, ts);
+    }
+    unc[k] = u;
+    if (u) atomicOr(&any_unc, 1);
+  }
+  __syncthreads();
+  if (!any_unc) return;   // block-uniform
+  const int cw = (jb.nw + chunks - 1) / chunks;
+  const int wlo = min(jb.nw, (int)blockIdx.y * cw), whi = min(jb.nw, wlo + cw);
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
+  float lb[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) lb[k] = -INFINITY;
+  for (int c = wlo + wave; c < whi; c += 4) {
     const int j = c * 64 + lane;
     if (j >= jb.n) continue;
     const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
@@ -364,45 +407,26 @@ __device__ __forceinline__ float front_logit_lb(const Job& jb, const float* __re
       r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
     }
     if (__float_as_int(r[15]) & 1) continue;
-    const PairBound pb = pair_bound(r, tg);
-    if (pb.plus && !pb.unbounded)
-      lb = fmaxf(lb, fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      if (!unc[k]) continue;   // block-uniform
+      const PairBound pb = pair_bound(r, tg[k]);
+      if (pb.plus && !pb.unbounded)
+        lb[k] = fmaxf(lb[k], fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
+    }
   }
-  return lb;
-}
-
-// Phase 2 of a chunked membership (chunks > 1): for every tile of a group whose phase-1 lmin is
-// -inf, the bound front_logit_lb over this block's chunk of the Gaussians, to
-// lbpart[(group * chunks + chunk) * kMaskMaxSub + tile of the group] (k_tile_mask takes the max).
-__global__ __launch_bounds__(256) void k_mask_lb(const Job* __restrict__ jobs, int n_jobs, int ts,
-                                                 const float* __restrict__ rec, const float* __restrict__ tile_lmin,
-                                                 const int32_t* __restrict__ job_flags, int chunks,
-                                                 float* __restrict__ lbpart) {
-  const int64_t gs = blockIdx.x;
-  const int ji = find_job(n_jobs, gs, [&](int k) { return jobs[k].stile_base; });
-  const Job& jb = jobs[ji];
-  if ((job_flags[ji] & 1) != 0 || !bounds_apply(jb)) return;   // dense job: no thresholds
-  const int sf = jb.sf;
-  const int st = (int)(gs - jb.stile_base);
-  const int sx = st % jb.stiles_x, sy = st / jb.stiles_x;
-  const int tx0 = sx * sf, ty0 = sy * sf;
-  const int nsx = min(sf, jb.tiles_x - tx0), nsy = min(sf, jb.tiles_y - ty0);
-  const int cw = (jb.nw + chunks - 1) / chunks;
-  const int wlo = min(jb.nw, (int)blockIdx.y * cw), whi = min(jb.nw, wlo + cw);
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
-  __shared__ float sh_lb[4];
-  for (int k = 0; k < nsx * nsy; ++k) {
-    const int txk = tx0 + k % nsx, tyk = ty0 + k / nsx;
-    if (!(tile_lmin[jb.tile_base + (int64_t)tyk * jb.tiles_x + txk] == -INFINITY)) continue;   // block-uniform
-    float lb = front_logit_lb(jb, rec, tile_geo_xy(jb, txk, tyk, ts), wlo, whi, wave, lane, b3l, b2l);
-    for (int o = 32; o > 0; o >>= 1) lb = fmaxf(lb, __shfl_xor(lb, o, 64));
-    if (lane == 0) sh_lb[wave] = lb;
-    __syncthreads();
-    if (threadIdx.x == 0)
-      lbpart[((int64_t)gs * chunks + blockIdx.y) * kMaskMaxSub + k] =
-          fmaxf(fmaxf(sh_lb[0], sh_lb[1]), fmaxf(sh_lb[2], sh_lb[3]));
-    __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    if (!unc[k]) continue;   // block-uniform
+    float v = lb[k];
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if (lane == 0) sh_lb[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < nsub && unc[threadIdx.x]) {
+    const int k = threadIdx.x;
+    lbpart[((int64_t)gs * chunks + blockIdx.y) * kMaskMaxSub + k] =
+        fmaxf(fmaxf(sh_lb[0][k], sh_lb[1][k]), fmaxf(sh_lb[2][k], sh_lb[3][k]));
   }
 }
 
@@ -470,15 +494,50 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
       }
       __syncthreads();
     } else {
-      __shared__ float sh_lb[4];
-      for (int k = 0; k < nsub; ++k) {
-        if (!(lmin[k] == -INFINITY)) continue;   // block-uniform
-        float lb = front_logit_lb(jb, rec, tg[k], 0, jb.nw, wave, lane, b3l, b2l);
-        for (int o = 32; o > 0; o >>= 1) lb = fmaxf(lb, __shfl_xor(lb, o, 64));
-        if (lane == 0) sh_lb[wave] = lb;
+      // one pass over the Gaussians serves all such tiles of the group: each record is loaded
+      // once and bounded against each of them (the same values and maxima as a pass per tile)
+      __shared__ float sh_lb[4][NS];
+      __shared__ int any_unc;
+      if (threadIdx.x == 0) {
+        int u = 0;
+        for (int k = 0; k < nsub; ++k) u |= (lmin[k] == -INFINITY) ? 1 : 0;
+        any_unc = u;
+      }
+      __syncthreads();
+      if (any_unc) {   // block-uniform
+        float lb[NS];
+#pragma unroll
+        for (int k = 0; k < NS; ++k) lb[k] = -INFINITY;
+        for (int c = wave; c < jb.nw; c += 4) {
+          const int j = c * 64 + lane;
+          if (j >= jb.n) continue;
+          const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
+          float r[16];
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float4 t4 = r4[i];
+            r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
+          }
+          if (__float_as_int(r[15]) & 1) continue;
+#pragma unroll
+          for (int k = 0; k < NS; ++k) {
+            if (k >= nsub || !(lmin[k] == -INFINITY)) continue;   // block-uniform
+            const PairBound pb = pair_bound(r, tg[k]);
+            if (pb.plus && !pb.unbounded)
+              lb[k] = fmaxf(lb[k], fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+          if (k >= nsub || !(lmin[k] == -INFINITY)) continue;   // block-uniform
+          float v = lb[k];
+          for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+          if (lane == 0) sh_lb[wave][k] = v;
+        }
         __syncthreads();
-        if (threadIdx.x == 0) {
-          const float m = fmaxf(fmaxf(sh_lb[0], sh_lb[1]), fmaxf(sh_lb[2], sh_lb[3]));
+        if (threadIdx.x < nsub && lmin[threadIdx.x] == -INFINITY) {
+          const int k = threadIdx.x;
+          const float m = fmaxf(fmaxf(sh_lb[0][k], sh_lb[1][k]), fmaxf(sh_lb[2][k], sh_lb[3][k]));
           lmin[k] = m;
           tau[k] = m - jb.delta2;
         }
@@ -1661,7 +1720,11 @@ hipError_t launch_tile_mask(hipStream_t s, int phase, const Job* jobs, int n_job
     else hipLaunchKernelGGL((k_tile_mask<1, 1>), g, b, 0, s, FE_MASK_ARGS);
     if (chunks > 1) hipLaunchKernelGGL(k_mask_finish<1>, dim3((unsigned)total_stiles), dim3(64), 0, s, FE_FINISH_ARGS);
   } else {
-    if (chunks > 1) hipLaunchKernelGGL(k_mask_lb, g, b, 0, s, jobs, n_jobs, ts, rec, tile_lmin, job_flags, chunks, lbpart);
+    if (chunks > 1) {
+      if (ts == 8) hipLaunchKernelGGL(k_mask_lb<4>, g, b, 0, s, jobs, n_jobs, ts, rec, tile_lmin, job_flags, chunks, lbpart);
+      else if (ts == 16) hipLaunchKernelGGL(k_mask_lb<3>, g, b, 0, s, jobs, n_jobs, ts, rec, tile_lmin, job_flags, chunks, lbpart);
+      else hipLaunchKernelGGL(k_mask_lb<1>, g, b, 0, s, jobs, n_jobs, ts, rec, tile_lmin, job_flags, chunks, lbpart);
+    }
     if (ts == 8) hipLaunchKernelGGL((k_tile_mask<2, 4>), g, b, 0, s, FE_MASK_ARGS);
     else if (ts == 16) hipLaunchKernelGGL((k_tile_mask<2, 3>), g, b, 0, s, FE_MASK_ARGS);
     else hipLaunchKernelGGL((k_tile_mask<2, 1>), g, b, 0, s, FE_MASK_ARGS);
