@@ -347,12 +347,53 @@ __device__ void finish_tile(const Job& jb, int64_t tile, uint32_t total, uint32_
   }
 }
 
+// Per lane, over the Gaussians of words [w0, w1) (lane = Gaussian, waves interleaved), for each
+// tile k of the group with need(k) (block-uniform), the largest lower bound
+// b3 (w - sd_hi^2/2) - b2 t_ub of the logit at every pixel of tile tg[k] of a Gaussian entirely
+// in front of its rays (-inf if none), into lb[k]; then the block's maxima into sh[wave][k].
+// One pass serves all such tiles: each record is loaded once and bounded against each of them
+// (the same values, and maxima, as a pass per tile).
+template <int NS, typename Need>
+__device__ __forceinline__ void front_logit_lbs(const Job& jb, const float* __restrict__ rec, const TileGeo* tg,
+                                                Need need, int w0, int w1, int wave, int lane, float b3l,
+                                                float b2l, float (*sh)[NS]) {
+  float lb[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) lb[k] = -INFINITY;
+  for (int c = w0 + wave; c < w1; c += 4) {
+    const int j = c * 64 + lane;
+    if (j >= jb.n) continue;
+    const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
+    float r[16];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const float4 t4 = r4[i];
+      r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
+    }
+    if (__float_as_int(r[15]) & 1) continue;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+      if (!need(k)) continue;   // block-uniform
+      const PairBound pb = pair_bound(r, tg[k]);
+      if (pb.plus && !pb.unbounded)
+        lb[k] = fmaxf(lb[k], fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    if (!need(k)) continue;   // block-uniform
+    float v = lb[k];
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    if (lane == 0) sh[wave][k] = v;
+  }
+  __syncthreads();
+}
+
 // Phase 2 of a chunked membership (chunks > 1): for every tile of a group whose phase-1 lmin is
 // -inf, the largest lower bound b3 (w - sd_hi^2/2) - b2 t_ub of the logit at every pixel of the
 // tile over the Gaussians of this block's chunk that lie entirely in front of the tile's rays
 // (-inf if none), to lbpart[(group * chunks + chunk) * kMaskMaxSub + tile of the group]
-// (k_tile_mask takes the max over the chunks).  One pass over the chunk serves all of the
-// group's such tiles: each record is loaded once and bounded against each of them.
+// (k_tile_mask takes the max over the chunks).
 template <int SFMAX>
 __global__ __launch_bounds__(256) void k_mask_lb(const Job* __restrict__ jobs, int n_jobs, int ts,
                                                  const float* __restrict__ rec, const float* __restrict__ tile_lmin,
@@ -393,36 +434,7 @@ __global__ __launch_bounds__(256) void k_mask_lb(const Job* __restrict__ jobs, i
   const int wlo = min(jb.nw, (int)blockIdx.y * cw), whi = min(jb.nw, wlo + cw);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const float b3l = jb.beta3 * kLog2e, b2l = jb.beta2 * kLog2e;
-  float lb[NS];
-#pragma unroll
-  for (int k = 0; k < NS; ++k) lb[k] = -INFINITY;
-  for (int c = wlo + wave; c < whi; c += 4) {
-    const int j = c * 64 + lane;
-    if (j >= jb.n) continue;
-    const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
-    float r[16];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const float4 t4 = r4[i];
-      r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
-    }
-    if (__float_as_int(r[15]) & 1) continue;
-#pragma unroll
-    for (int k = 0; k < NS; ++k) {
-      if (!unc[k]) continue;   // block-uniform
-      const PairBound pb = pair_bound(r, tg[k]);
-      if (pb.plus && !pb.unbounded)
-        lb[k] = fmaxf(lb[k], fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < NS; ++k) {
-    if (!unc[k]) continue;   // block-uniform
-    float v = lb[k];
-    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    if (lane == 0) sh_lb[wave][k] = v;
-  }
-  __syncthreads();
+  front_logit_lbs<NS>(jb, rec, tg, [&](int k) { return unc[k] != 0; }, wlo, whi, wave, lane, b3l, b2l, sh_lb);
   if (threadIdx.x < nsub && unc[threadIdx.x]) {
     const int k = threadIdx.x;
     lbpart[((int64_t)gs * chunks + blockIdx.y) * kMaskMaxSub + k] =
@@ -494,8 +506,6 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
       }
       __syncthreads();
     } else {
-      // one pass over the Gaussians serves all such tiles of the group: each record is loaded
-      // once and bounded against each of them (the same values and maxima as a pass per tile)
       __shared__ float sh_lb[4][NS];
       __shared__ int any_unc;
       if (threadIdx.x == 0) {
@@ -505,36 +515,8 @@ __global__ __launch_bounds__(256, (PHASE == 2 && SFMAX == 3) ? 4 : 1) void k_til
       }
       __syncthreads();
       if (any_unc) {   // block-uniform
-        float lb[NS];
-#pragma unroll
-        for (int k = 0; k < NS; ++k) lb[k] = -INFINITY;
-        for (int c = wave; c < jb.nw; c += 4) {
-          const int j = c * 64 + lane;
-          if (j >= jb.n) continue;
-          const float4* r4 = reinterpret_cast<const float4*>(rec + (jb.rec_base + j) * kRecFloats);
-          float r[16];
-#pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const float4 t4 = r4[i];
-            r[4 * i] = t4.x; r[4 * i + 1] = t4.y; r[4 * i + 2] = t4.z; r[4 * i + 3] = t4.w;
-          }
-          if (__float_as_int(r[15]) & 1) continue;
-#pragma unroll
-          for (int k = 0; k < NS; ++k) {
-            if (k >= nsub || !(lmin[k] == -INFINITY)) continue;   // block-uniform
-            const PairBound pb = pair_bound(r, tg[k]);
-            if (pb.plus && !pb.unbounded)
-              lb[k] = fmaxf(lb[k], fmaf(b3l, fmaf(-0.5f * pb.sd_hi, pb.sd_hi, r[12]), -(b2l * pair_t_ub(pb))));
-          }
-        }
-#pragma unroll
-        for (int k = 0; k < NS; ++k) {
-          if (k >= nsub || !(lmin[k] == -INFINITY)) continue;   // block-uniform
-          float v = lb[k];
-          for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-          if (lane == 0) sh_lb[wave][k] = v;
-        }
-        __syncthreads();
+        front_logit_lbs<NS>(jb, rec, tg, [&](int k) { return k < nsub && lmin[k] == -INFINITY; }, 0, jb.nw, wave,
+                            lane, b3l, b2l, sh_lb);
         if (threadIdx.x < nsub && lmin[threadIdx.x] == -INFINITY) {
           const int k = threadIdx.x;
           const float m = fmaxf(fmaxf(sh_lb[0][k], sh_lb[1][k]), fmaxf(sh_lb[2][k], sh_lb[3][k]));
